@@ -1,0 +1,247 @@
+"""Benchmark: HBM GB/s per fused gate on a 30-qubit f64 statevector.
+
+Workload (one GPU): the 30-qubit random-layer circuit (depth 20; each layer =
+Haar SU(2) on every qubit, then CZ on a seeded random perfect matching),
+fused to k <= 5 dense gates by the hardware-adaptive cost model, applied to
+one resident 16 GiB f64 statevector.  One "step" = one pass of the whole
+fused circuit.  value = fused gates x algorithmic bytes per gate / wall time
+of the timed steps, where the algorithmic bytes of a gate on n qubits are
+2 * 2^n * 2 * 8 (one read and one write of every amplitude).
+
+Also reported: the dominant kernel's HBM roofline fraction from HIP events on
+the launch stream, the f32 figure of the same circuit, gate-updates/s, and
+the CPU baseline (the oracle's SIMD split-layout path on this host's cores,
+timed on a bounded sample of the same fused gates).
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--qubits 30]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from collections import defaultdict
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "HBM GB/s per fused gate at 30q f64; batched circuits/s + p99 at fixed fidelity"
+HBM_PEAK_GBS = 8000.0
+SEED = 20260101
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def hw_profile(cs, precision: str):
+    """Cost-model profile: the measured MI355X refit when present, else the
+    datasheet roofline (8 TB/s; FP64 78.6 TF, FP32 VALU 78.6 TF without
+    packed FP32)."""
+    path = ROOT / "cast_simulator_amd" / "profiles" / f"mi355x_{precision}.json"
+    if path.exists():
+        return cs.HardwareProfile.load(str(path)), str(path.relative_to(ROOT))
+    cfg = cs.ProfileConfig()
+    cfg.name, cfg.arch, cfg.precision, cfg.n_qubits = "MI355X (datasheet)", "gfx950", precision, 30
+    sb = 8 if precision == "f64" else 4
+    # reference GFLOP/s unit = half the true flops (src/profile.rs:436)
+    peak_ref_gflops = 78.6e12 / 2 / 1e9
+    return cs.HardwareProfile.from_roofline(cfg, HBM_PEAK_GBS * 1e9 / 2**30, peak_ref_gflops, sb), \
+        "datasheet"
+
+
+def fused_circuit(cs, n: int, precision: str, max_size: int):
+    prof, src = hw_profile(cs, precision)
+    ztol = 1e-12 if precision == "f64" else 1e-6
+    g = cs.CircuitGraph.from_circuit(cs.random_layer_circuit(n, 20, SEED))
+    cs.optimize(g, cs.FusionConfig.hardware_adaptive(prof, max_size, ztol))
+    return g.gates_in_row_order(), prof, src
+
+
+def run_gpu(cs, n: int, precision: str, steps: int, warmup: int, max_size: int):
+    gates, prof, src = fused_circuit(cs, n, precision, max_size)
+    spec = cs.HipKernelGenSpec.f64() if precision == "f64" else cs.HipKernelGenSpec.f32()
+    mgr = cs.HipKernelManager(spec)
+    ids = [mgr.generate(g) for g in gates]
+    info = {i: mgr.plan_info(i) for i in set(ids)}
+    sv = cs.HipStatevector(n, cs.Precision.F64 if precision == "f64" else cs.Precision.F32)
+    sv.zero()
+    for s in range(warmup):
+        for i in ids:
+            mgr.apply(i, sv)
+        mgr.sync(False)
+        log(f"  warmup step {s + 1}/{warmup} done")
+    mgr.sync(False)
+    per_kernel = defaultdict(list)
+    t0 = time.perf_counter()
+    for s in range(steps):
+        for i in ids:
+            mgr.apply(i, sv)
+        kernels, _ = mgr.sync(True)
+        for kid, t in kernels:
+            p = info[kid]
+            per_kernel[(p.k, p.variant)].append(t)
+        log(f"  timed step {s + 1}/{steps} done")
+    wall = time.perf_counter() - t0
+    scalar = 8 if precision == "f64" else 4
+    bytes_per_gate = 2 * (1 << n) * 2 * scalar
+    n_gates = len(ids)
+    gbs = n_gates * steps * bytes_per_gate / wall / 1e9
+    # dominant kernel = the (k, variant) group with the most GPU time
+    (kdom, vdom), times = max(per_kernel.items(), key=lambda kv: sum(kv[1]))
+    avg = sum(times) / len(times)
+    achieved = bytes_per_gate / avg / 1e9
+    norm = sv.norm_squared()
+    hist = defaultdict(int)
+    for i in ids:
+        hist[info[i].k] += 1
+    return {
+        "gates": gates,
+        "n_gates": n_gates,
+        "k_hist": dict(sorted(hist.items())),
+        "wall_s": wall,
+        "gbs_per_gate": gbs,
+        "ms_per_step": wall / steps * 1e3,
+        "gate_updates_per_s": n_gates * steps / wall,
+        "dominant": {"k": kdom, "variant": vdom, "launches": len(times), "avg_ms": avg * 1e3,
+                     "achieved_gbs": achieved},
+        "per_k_gbs": {f"k{k}v{v}": bytes_per_gate / (sum(t) / len(t)) / 1e9
+                      for (k, v), t in sorted(per_kernel.items())},
+        "norm_after": norm,
+        "profile": src,
+        "crossover_ai": prof.crossover_ai,
+    }
+
+
+def cpu_baseline(gates, n_gpu: int, budget_s: float = 20.0):
+    """The oracle's SIMD split-layout kernels (a restatement of the
+    reference CPU path) on this host's logical CPUs, over the first fused
+    gates of the same circuit, on the same qubit count when host memory
+    allows (else 28 qubits)."""
+    import numpy as np
+
+    import oracle
+
+    avail = 0
+    try:
+        for line in open("/proc/meminfo"):
+            if line.startswith("MemAvailable:"):
+                avail = int(line.split()[1]) * 1024
+    except OSError:
+        pass
+    n = n_gpu if avail > 3 * (2 << n_gpu) * 8 else 28
+    threads = oracle.num_cpus()
+    sv = oracle.SimdStatevector(n, f32=False)
+    sv.buf[0] = 1.0
+    warm = gates[0]
+    sv.apply(np.asarray(warm.matrix()), list(warm.qubits()), threads)  # page-in, untimed
+    done, t0 = 0, time.perf_counter()
+    for g in gates[1:]:
+        if max(g.qubits()) >= n:
+            continue
+        sv.apply(np.asarray(g.matrix()), list(g.qubits()), threads)
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    b = 2 * (1 << n) * 2 * 8
+    return {
+        "value": done * b / dt / 1e9,
+        "unit": "GB/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{done} fused gates (k<=5) of the same circuit on a {n}-qubit f64 split-layout "
+                  f"statevector, {threads} threads, {dt:.1f} s",
+        "cpu": platform.processor() or platform.machine(),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--qubits", type=int, default=30)
+    ap.add_argument("--max-size", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-f32", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); the "
+              "multi-GPU design is described in DESIGN.md and is not run here.", file=sys.stderr)
+        return 2
+
+    import cast_simulator_amd as cs
+
+    log(f"[bench] f64 {args.qubits}q circuit: warmup {args.warmup}, steps {args.steps}")
+    r64 = run_gpu(cs, args.qubits, "f64", args.steps, args.warmup, args.max_size)
+    log(f"[bench] f64: {r64['gbs_per_gate']:.1f} GB/s per gate, {r64['n_gates']} gates, "
+        f"{r64['ms_per_step']:.1f} ms/step, per-k {r64['per_k_gbs']}")
+    r32 = None
+    if not args.no_f32:
+        r32 = run_gpu(cs, args.qubits, "f32", args.steps, args.warmup, args.max_size)
+        log(f"[bench] f32: {r32['gbs_per_gate']:.1f} GB/s per gate, per-k {r32['per_k_gbs']}")
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] cpu baseline ...")
+        cpu = cpu_baseline(r64["gates"], args.qubits)
+        log(f"[bench] cpu: {cpu['value']:.2f} GB/s ({cpu['sample']})")
+
+    dom = r64["dominant"]
+    line = {
+        "metric": METRIC,
+        "value": round(r64["gbs_per_gate"], 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r64["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.qubits}-qubit random-layer circuit, depth 20, seed {SEED}, "
+                        f"fused to k<={args.max_size} by the hardware-adaptive cost model, "
+                        "one resident f64 statevector",
+            "qubits": args.qubits,
+            "fused_gates": r64["n_gates"],
+            "k_histogram": r64["k_hist"],
+            "cost_model_profile": r64["profile"],
+            "crossover_ai": round(r64["crossover_ai"], 2),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": f"k_dense k={dom['k']} f64" if dom["variant"] == 1 else
+                      f"variant {dom['variant']} k={dom['k']} f64",
+            "achieved": round(dom["achieved_gbs"], 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(dom["achieved_gbs"] / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "avg_launch_ms": round(dom["avg_ms"], 4),
+            "launches": dom["launches"],
+        },
+        "per_kernel_gbs": {k: round(v, 1) for k, v in r64["per_k_gbs"].items()},
+        "gate_updates_per_s": round(r64["gate_updates_per_s"], 2),
+        "amplitude_updates_per_s": round(r64["gate_updates_per_s"] * (1 << args.qubits), 0),
+        "cpu_baseline": cpu,
+    }
+    if r32:
+        line["f32"] = {
+            "gbs_per_gate": round(r32["gbs_per_gate"], 2),
+            "ms_per_step": round(r32["ms_per_step"], 3),
+            "per_kernel_gbs": {k: round(v, 1) for k, v in r32["per_k_gbs"].items()},
+            "fused_gates": r32["n_gates"],
+        }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,270 @@
+"""ctypes view of the C ABI in ``include/cast_hip.h``.
+
+This is the binding a foreign host (the reference's Rust crate would use
+``extern "C"`` declarations, src/ffi/cuda.rs) sees: plain pointers, sizes,
+uint64 device addresses, opaque handles and (return code, err_buf) errors.
+Tests use it to drive the kernels directly through the boundary.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+
+HEADER = Path(__file__).resolve().parent.parent / "include" / "cast_hip.h"
+LIB_PATH = Path(__file__).resolve().parent / "lib" / "libcast_hip.so"
+ERR_LEN = 1024
+
+F32, F64 = 0, 1
+VARIANT_AUTO, VARIANT_DENSE, VARIANT_MONOMIAL, VARIANT_LDS, VARIANT_MFMA = 0, 1, 2, 3, 4
+
+
+class AbiError(RuntimeError):
+    """A cast_hip_* call returned failure; the message is the library's err_buf."""
+
+
+class Complex64(C.Structure):
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+
+class KernelGenSpec(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int),
+        ("ztol", C.c_double),
+        ("otol", C.c_double),
+        ("variant", C.c_uint32),
+        ("reserved", C.c_uint32),
+    ]
+
+
+class PlanInfo(C.Structure):
+    _fields_ = [
+        ("n_qubits", C.c_uint32),
+        ("precision", C.c_uint32),
+        ("variant", C.c_uint32),
+        ("nnz_rows", C.c_uint32),
+        ("scalar_nnz", C.c_uint64),
+        ("qubits", C.c_uint32 * 8),
+    ]
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [
+        ("name", C.c_char * 128),
+        ("arch", C.c_char * 64),
+        ("n_cu", C.c_uint32),
+        ("wavefront_size", C.c_uint32),
+        ("lds_per_block", C.c_uint32),
+        ("l2_bytes", C.c_uint32),
+        ("clock_khz", C.c_uint32),
+        ("mem_clock_khz", C.c_uint32),
+        ("mem_bus_width", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("total_mem", C.c_uint64),
+    ]
+
+
+_vp, _sz, _u8, _u32, _u64, _i = C.c_void_p, C.c_size_t, C.c_uint8, C.c_uint32, C.c_uint64, C.c_int
+_err = C.c_char_p
+_SIGS = {
+    "cast_hip_version": (C.c_char_p, []),
+    "cast_hip_device_info": (_i, [C.POINTER(DeviceInfo), _err, _sz]),
+    "cast_hip_free_memory": (_i, [C.POINTER(_u64), C.POINTER(_u64), _err, _sz]),
+    "cast_hip_stream_create": (_vp, [_err, _sz]),
+    "cast_hip_stream_destroy": (None, [_vp]),
+    "cast_hip_stream_sync": (_i, [_vp, _err, _sz]),
+    "cast_hip_event_create": (_vp, [_err, _sz]),
+    "cast_hip_event_record": (_i, [_vp, _vp, _err, _sz]),
+    "cast_hip_event_synchronize": (_i, [_vp, _err, _sz]),
+    "cast_hip_event_destroy": (None, [_vp]),
+    "cast_hip_event_elapsed_ms": (_i, [_vp, _vp, C.POINTER(C.c_float), _err, _sz]),
+    "cast_hip_memcpy_dtod_async": (_i, [_u64, _u64, _sz, _vp, _err, _sz]),
+    "cast_hip_sv_alloc": (_u64, [_sz, _u8, _err, _sz]),
+    "cast_hip_sv_free": (None, [_u64]),
+    "cast_hip_sv_zero": (_i, [_u64, _sz, _u8, _err, _sz]),
+    "cast_hip_sv_upload": (_i, [_u64, C.POINTER(C.c_double), _sz, _u8, _err, _sz]),
+    "cast_hip_sv_download": (_i, [_u64, C.POINTER(C.c_double), _sz, _u8, _err, _sz]),
+    "cast_hip_norm_squared": (_i, [_u64, _sz, _u8, C.POINTER(C.c_double), _err, _sz]),
+    "cast_hip_scale": (_i, [_u64, _sz, _u8, C.c_double, _err, _sz]),
+    "cast_hip_gate_plan_create": (
+        _i,
+        [C.POINTER(KernelGenSpec), C.POINTER(Complex64), _sz, C.POINTER(_u32), _sz,
+         C.POINTER(_vp), _err, _sz],
+    ),
+    "cast_hip_gate_plan_destroy": (None, [_vp]),
+    "cast_hip_gate_plan_info": (_i, [_vp, C.POINTER(PlanInfo), _err, _sz]),
+    "cast_hip_gate_apply": (_i, [_vp, _vp, _u64, _u32, _err, _sz]),
+    "cast_hip_gate_apply_batch": (
+        _i, [C.POINTER(_vp), C.POINTER(_u64), C.POINTER(_u32), _sz, _vp, _err, _sz]),
+    "cast_hip_expect_zz": (
+        _i, [_u64, _u32, _u8, C.POINTER(_u32), _sz, C.POINTER(C.c_double), _vp, _err, _sz]),
+}
+
+
+def declared_symbols(header: Path = HEADER) -> list[str]:
+    """Function names declared in a C header of this repository."""
+    text = re.sub(r"/\*.*?\*/", "", header.read_text(), flags=re.S)
+    return sorted(set(re.findall(r"\b(cast_[a-z0-9_]+)\s*\(", text)))
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} is missing: build the gfx950 backend first")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _call(name: str, *args):
+    buf = C.create_string_buffer(ERR_LEN)
+    rc = getattr(lib(), name)(*args, buf, ERR_LEN)
+    if rc != 0:
+        raise AbiError(f"{name}: {buf.value.decode(errors='replace')}")
+
+
+def _ctor(name: str, *args):
+    buf = C.create_string_buffer(ERR_LEN)
+    h = getattr(lib(), name)(*args, buf, ERR_LEN)
+    if not h:
+        raise AbiError(f"{name}: {buf.value.decode(errors='replace')}")
+    return h
+
+
+def device_info() -> DeviceInfo:
+    d = DeviceInfo()
+    _call("cast_hip_device_info", C.byref(d))
+    return d
+
+
+def free_memory() -> tuple[int, int]:
+    f, t = _u64(), _u64()
+    _call("cast_hip_free_memory", C.byref(f), C.byref(t))
+    return f.value, t.value
+
+
+def sv_alloc(n_qubits: int, precision: int) -> int:
+    return _ctor("cast_hip_sv_alloc", 2 << n_qubits, precision)
+
+
+def sv_free(dptr: int) -> None:
+    lib().cast_hip_sv_free(dptr)
+
+
+def sv_zero(dptr: int, n_qubits: int, precision: int) -> None:
+    _call("cast_hip_sv_zero", dptr, 2 << n_qubits, precision)
+
+
+def sv_upload(dptr: int, host: np.ndarray, precision: int) -> None:
+    h = np.ascontiguousarray(host, dtype=np.float64).reshape(-1)
+    _call("cast_hip_sv_upload", dptr, h.ctypes.data_as(C.POINTER(C.c_double)), h.size, precision)
+
+
+def sv_download(dptr: int, n_qubits: int, precision: int) -> np.ndarray:
+    h = np.empty(2 << n_qubits, dtype=np.float64)
+    _call("cast_hip_sv_download", dptr, h.ctypes.data_as(C.POINTER(C.c_double)), h.size,
+          precision)
+    return h
+
+
+def norm_squared(dptr: int, n_qubits: int, precision: int) -> float:
+    out = C.c_double()
+    _call("cast_hip_norm_squared", dptr, 2 << n_qubits, precision, C.byref(out))
+    return out.value
+
+
+def scale(dptr: int, n_qubits: int, precision: int, factor: float) -> None:
+    _call("cast_hip_scale", dptr, 2 << n_qubits, precision, factor)
+
+
+def plan_create(matrix: np.ndarray, qubits, precision: int, ztol: float | None = None,
+                otol: float | None = None, variant: int = VARIANT_AUTO) -> int:
+    m = np.ascontiguousarray(matrix, dtype=np.complex128).reshape(-1)
+    arr = (Complex64 * m.size)(*[(z.real, z.imag) for z in m])
+    q = (_u32 * len(qubits))(*qubits)
+    tol = 1e-6 if precision == F32 else 1e-12
+    spec = KernelGenSpec(precision, tol if ztol is None else ztol, tol if otol is None else otol,
+                         variant, 0)
+    out = _vp()
+    _call("cast_hip_gate_plan_create", C.byref(spec), arr, m.size, q, len(qubits), C.byref(out))
+    return out.value
+
+
+def plan_destroy(plan: int) -> None:
+    lib().cast_hip_gate_plan_destroy(plan)
+
+
+def plan_info(plan: int) -> PlanInfo:
+    info = PlanInfo()
+    _call("cast_hip_gate_plan_info", plan, C.byref(info))
+    return info
+
+
+def gate_apply(plan: int, dptr: int, n_qubits: int, stream: int | None = None) -> None:
+    _call("cast_hip_gate_apply", plan, stream, dptr, n_qubits)
+
+
+def gate_apply_batch(plans, svs, n_qubits, stream: int | None = None) -> None:
+    n = len(plans)
+    _call("cast_hip_gate_apply_batch", (_vp * n)(*plans), (_u64 * n)(*svs),
+          (_u32 * n)(*n_qubits), n, stream)
+
+
+def expect_zz(dptr: int, n_qubits: int, precision: int, pairs, stream: int | None = None):
+    flat = [q for p in pairs for q in p]
+    out = (C.c_double * len(pairs))()
+    _call("cast_hip_expect_zz", dptr, n_qubits, precision, (_u32 * len(flat))(*flat),
+          len(pairs), out, stream)
+    return np.array(out[:], dtype=np.float64)
+
+
+def stream_create() -> int:
+    return _ctor("cast_hip_stream_create")
+
+
+def stream_destroy(s: int) -> None:
+    lib().cast_hip_stream_destroy(s)
+
+
+def stream_sync(s: int | None) -> None:
+    _call("cast_hip_stream_sync", s)
+
+
+def event_create() -> int:
+    return _ctor("cast_hip_event_create")
+
+
+def event_record(e: int, s: int | None) -> None:
+    _call("cast_hip_event_record", e, s)
+
+
+def event_synchronize(e: int) -> None:
+    _call("cast_hip_event_synchronize", e)
+
+
+def event_elapsed_ms(a: int, b: int) -> float:
+    out = C.c_float()
+    _call("cast_hip_event_elapsed_ms", a, b, C.byref(out))
+    return out.value
+
+
+def event_destroy(e: int) -> None:
+    lib().cast_hip_event_destroy(e)
+
+
+def memcpy_dtod_async(dst: int, src: int, n_bytes: int, stream: int | None = None) -> None:
+    _call("cast_hip_memcpy_dtod_async", dst, src, n_bytes, stream)
+
+
+def version() -> str:
+    return lib().cast_hip_version().decode()

@@ -1,0 +1,495 @@
+// C ABI entry points of libcast_hip.so (declared in include/cast_hip.h).
+//
+// Every entry point is an exception firewall: implementation errors are
+// C++ exceptions that are caught here and turned into (return 1 / null,
+// err_buf message), as src/cpp/cuda/ffi_cuda.cpp:138-144 does.
+
+#include "cast_hip.h"
+#include "common.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <vector>
+
+namespace cast_hip {
+
+void ensure_device() {
+  int n = 0;
+  const hipError_t rc = hipGetDeviceCount(&n);
+  if (rc != hipSuccess || n <= 0)
+    throw Error(std::string("no HIP device available (hipGetDeviceCount: ") +
+                hipGetErrorString(rc) + ")");
+}
+
+const DeviceProps &device_props() {
+  static std::once_flag once;
+  static DeviceProps props;
+  std::call_once(once, [] {
+    ensure_device();
+    int dev = 0;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    hipDeviceProp_t p{};
+    check(hipGetDeviceProperties(&p, dev), "hipGetDeviceProperties");
+    props.device = dev;
+    props.n_cu = p.multiProcessorCount;
+    props.warp = p.warpSize;
+    props.lds_per_block = p.sharedMemPerBlock;
+    props.l2_bytes = p.l2CacheSize;
+    props.arch = p.gcnArchName;
+    props.name = p.name;
+  });
+  return props;
+}
+
+namespace {
+
+template <typename F> int guard(char *err, size_t len, F &&f) {
+  try {
+    f();
+    clear_err(err, len);
+    return 0;
+  } catch (const std::exception &e) {
+    write_err(err, len, e.what());
+  } catch (...) {
+    write_err(err, len, "unknown C++ exception");
+  }
+  return 1;
+}
+
+void check_sv_args(uint64_t dptr, size_t n_elements, uint8_t precision) {
+  check_precision(precision);
+  if (dptr == 0)
+    throw Error("null device pointer");
+  if (n_elements == 0)
+    throw Error("n_elements must be positive");
+}
+
+// Matrix scalar classification of src/cpp/cuda/cuda_gen.cpp:67-112: the
+// tolerances are divided by K, zeros are tested first, then +1, then -1.
+double snap(double v, double ztol, double otol, double K, bool fp32) {
+  if (ztol > 0.0 && std::fabs(v) < ztol / K)
+    return 0.0;
+  if (otol > 0.0 && std::fabs(v - 1.0) < otol / K)
+    return 1.0;
+  if (otol > 0.0 && std::fabs(v + 1.0) < otol / K)
+    return -1.0;
+  return fp32 ? double(float(v)) : v;
+}
+
+GatePlan *build_plan(const cast_hip_kernel_gen_spec_t *spec, const cast_complex64_t *matrix,
+                     size_t matrix_len, const uint32_t *qubits, size_t n_qubits) {
+  if (spec == nullptr)
+    throw Error("spec must not be null");
+  if (matrix == nullptr)
+    throw Error("matrix pointer must not be null");
+  if (qubits == nullptr || n_qubits == 0)
+    throw Error("qubits must not be null/empty");
+  if (n_qubits > 6)
+    throw Error("gate size k=" + std::to_string(n_qubits) + " exceeds the maximum of 6");
+  const size_t K = size_t(1) << n_qubits;
+  if (matrix_len != K * K)
+    throw Error("matrix_len must equal (2^n_qubits)^2");
+  if (spec->precision != CAST_PRECISION_F32 && spec->precision != CAST_PRECISION_F64)
+    throw Error("spec.precision must be F32 or F64");
+  for (size_t b = 0; b < n_qubits; ++b) {
+    if (qubits[b] > 40)
+      throw Error("qubit index " + std::to_string(qubits[b]) + " out of range");
+    if (b > 0 && qubits[b] <= qubits[b - 1])
+      throw Error("qubits must be strictly ascending");
+  }
+  if (spec->variant > CAST_HIP_VARIANT_MFMA)
+    throw Error("unknown variant " + std::to_string(spec->variant));
+
+  auto plan = std::make_unique<GatePlan>();
+  plan->k = uint32_t(n_qubits);
+  plan->precision = uint8_t(spec->precision);
+  for (size_t b = 0; b < n_qubits; ++b)
+    plan->q[b] = qubits[b];
+  const bool fp32 = spec->precision == CAST_PRECISION_F32;
+
+  std::vector<double> m(2 * K * K);
+  uint64_t nnz = 0;
+  for (size_t i = 0; i < K * K; ++i) {
+    m[2 * i] = snap(matrix[i].re, spec->ztol, spec->otol, double(K), fp32);
+    m[2 * i + 1] = snap(matrix[i].im, spec->ztol, spec->otol, double(K), fp32);
+    nnz += (m[2 * i] != 0.0) + (m[2 * i + 1] != 0.0);
+  }
+  plan->scalar_nnz = nnz;
+
+  // Structure: monomial when every row has at most one nonzero entry.
+  bool monomial = true;
+  uint32_t n_active = 0;
+  for (size_t r = 0; r < K && monomial; ++r) {
+    int col = -1;
+    for (size_t c = 0; c < K; ++c) {
+      if (m[2 * (r * K + c)] != 0.0 || m[2 * (r * K + c) + 1] != 0.0) {
+        if (col >= 0) {
+          monomial = false;
+          break;
+        }
+        col = int(c);
+      }
+    }
+    if (!monomial)
+      break;
+    double re = 0.0, im = 0.0;
+    if (col >= 0) {
+      re = m[2 * (r * K + col)];
+      im = m[2 * (r * K + col) + 1];
+    } else {
+      col = int(r); // all-zero row: output 0
+    }
+    const bool identity_row = size_t(col) == r && re == 1.0 && im == 0.0;
+    if (!identity_row) {
+      plan->act_row[n_active] = uint8_t(r);
+      plan->act_col[n_active] = uint8_t(col);
+      plan->act_re[n_active] = re;
+      plan->act_im[n_active] = im;
+      ++n_active;
+    }
+  }
+  plan->n_active = n_active;
+
+  uint32_t variant = spec->variant;
+  if (variant == CAST_HIP_VARIANT_AUTO)
+    variant = monomial ? CAST_HIP_VARIANT_MONOMIAL : CAST_HIP_VARIANT_DENSE;
+  if (variant == CAST_HIP_VARIANT_MONOMIAL && !monomial)
+    throw Error("variant MONOMIAL requested for a matrix with >1 nonzero in a row");
+  if (variant == CAST_HIP_VARIANT_LDS || variant == CAST_HIP_VARIANT_MFMA)
+    variant = CAST_HIP_VARIANT_DENSE; // not yet specialised: dense handles every matrix
+  plan->variant = variant;
+
+  plan->h_mat = new double[2 * K * K];
+  std::copy(m.begin(), m.end(), plan->h_mat);
+
+  ensure_device();
+  check(hipGetDevice(&plan->device), "hipGetDevice");
+  const size_t bytes = 2 * K * K * (fp32 ? 4 : 8);
+  check(hipMalloc(&plan->d_mat, bytes), "hipMalloc(gate matrix)");
+  if (fp32) {
+    std::vector<float> f(2 * K * K);
+    for (size_t i = 0; i < f.size(); ++i)
+      f[i] = float(m[i]);
+    check(hipMemcpy(plan->d_mat, f.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy(matrix)");
+  } else {
+    check(hipMemcpy(plan->d_mat, m.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy(matrix)");
+  }
+  return plan.release();
+}
+
+void free_plan(GatePlan *p) {
+  if (!p)
+    return;
+  if (p->d_mat)
+    (void)hipFree(p->d_mat);
+  delete[] p->h_mat;
+  delete p;
+}
+
+} // namespace
+} // namespace cast_hip
+
+using namespace cast_hip;
+
+extern "C" {
+
+const char *cast_hip_version(void) { return "cast_hip 0.1.0 (gfx950)"; }
+
+int cast_hip_device_info(cast_hip_device_info_t *out, char *err, size_t len) {
+  return guard(err, len, [&] {
+    if (!out)
+      throw Error("out must not be null");
+    ensure_device();
+    int dev = 0;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    hipDeviceProp_t p{};
+    check(hipGetDeviceProperties(&p, dev), "hipGetDeviceProperties");
+    std::memset(out, 0, sizeof(*out));
+    std::strncpy(out->name, p.name, sizeof(out->name) - 1);
+    std::strncpy(out->arch, p.gcnArchName, sizeof(out->arch) - 1);
+    out->n_cu = uint32_t(p.multiProcessorCount);
+    out->wavefront_size = uint32_t(p.warpSize);
+    out->lds_per_block = uint32_t(p.sharedMemPerBlock);
+    out->l2_bytes = uint32_t(p.l2CacheSize);
+    out->clock_khz = uint32_t(p.clockRate);
+    out->mem_clock_khz = uint32_t(p.memoryClockRate);
+    out->mem_bus_width = uint32_t(p.memoryBusWidth);
+    out->total_mem = uint64_t(p.totalGlobalMem);
+  });
+}
+
+int cast_hip_free_memory(uint64_t *out_free, uint64_t *out_total, char *err, size_t len) {
+  return guard(err, len, [&] {
+    if (!out_free || !out_total)
+      throw Error("output pointers must not be null");
+    ensure_device();
+    size_t f = 0, t = 0;
+    check(hipMemGetInfo(&f, &t), "hipMemGetInfo");
+    *out_free = f;
+    *out_total = t;
+  });
+}
+
+void *cast_hip_stream_create(char *err, size_t len) {
+  hipStream_t s = nullptr;
+  const int rc = guard(err, len, [&] {
+    ensure_device();
+    // Default priority, and blocking with respect to the null stream: the
+    // synchronous statevector calls (zero, copies, norm) run on the null
+    // stream and must stay ordered with gates launched here, as the
+    // reference relies on for clone_device (src/cuda/statevector.rs:232-252).
+    check(hipStreamCreateWithFlags(&s, hipStreamDefault), "hipStreamCreateWithFlags");
+  });
+  return rc == 0 ? static_cast<void *>(s) : nullptr;
+}
+
+void cast_hip_stream_destroy(void *stream) {
+  if (stream)
+    (void)hipStreamDestroy(static_cast<hipStream_t>(stream));
+}
+
+int cast_hip_stream_sync(void *stream, char *err, size_t len) {
+  return guard(err, len, [&] {
+    check(hipStreamSynchronize(static_cast<hipStream_t>(stream)), "hipStreamSynchronize");
+  });
+}
+
+void *cast_hip_event_create(char *err, size_t len) {
+  hipEvent_t e = nullptr;
+  const int rc = guard(err, len, [&] {
+    ensure_device();
+    check(hipEventCreate(&e), "hipEventCreate");
+  });
+  return rc == 0 ? static_cast<void *>(e) : nullptr;
+}
+
+int cast_hip_event_record(void *event, void *stream, char *err, size_t len) {
+  return guard(err, len, [&] {
+    if (!event)
+      throw Error("null event");
+    check(hipEventRecord(static_cast<hipEvent_t>(event), static_cast<hipStream_t>(stream)),
+          "hipEventRecord");
+  });
+}
+
+int cast_hip_event_synchronize(void *event, char *err, size_t len) {
+  return guard(err, len, [&] {
+    if (!event)
+      throw Error("null event");
+    check(hipEventSynchronize(static_cast<hipEvent_t>(event)), "hipEventSynchronize");
+  });
+}
+
+void cast_hip_event_destroy(void *event) {
+  if (event)
+    (void)hipEventDestroy(static_cast<hipEvent_t>(event));
+}
+
+int cast_hip_event_elapsed_ms(void *start, void *end, float *out_ms, char *err, size_t len) {
+  return guard(err, len, [&] {
+    if (!start || !end || !out_ms)
+      throw Error("null argument");
+    check(hipEventElapsedTime(out_ms, static_cast<hipEvent_t>(start),
+                              static_cast<hipEvent_t>(end)),
+          "hipEventElapsedTime");
+  });
+}
+
+int cast_hip_memcpy_dtod_async(uint64_t dst, uint64_t src, size_t n_bytes, void *stream,
+                               char *err, size_t len) {
+  return guard(err, len, [&] {
+    if (!dst || !src)
+      throw Error("null device pointer");
+    check(hipMemcpyAsync(reinterpret_cast<void *>(dst), reinterpret_cast<const void *>(src),
+                         n_bytes, hipMemcpyDeviceToDevice, static_cast<hipStream_t>(stream)),
+          "hipMemcpyAsync(DtoD)");
+  });
+}
+
+uint64_t cast_hip_sv_alloc(size_t n_elements, uint8_t precision, char *err, size_t len) {
+  void *p = nullptr;
+  const int rc = guard(err, len, [&] {
+    check_precision(precision);
+    if (n_elements == 0)
+      throw Error("n_elements must be positive");
+    ensure_device();
+    check(hipMalloc(&p, n_elements * scalar_bytes(precision)), "hipMalloc(statevector)");
+  });
+  return rc == 0 ? reinterpret_cast<uint64_t>(p) : 0;
+}
+
+void cast_hip_sv_free(uint64_t dptr) {
+  if (dptr)
+    (void)hipFree(reinterpret_cast<void *>(dptr));
+}
+
+int cast_hip_sv_zero(uint64_t dptr, size_t n_elements, uint8_t precision, char *err, size_t len) {
+  return guard(err, len, [&] {
+    check_sv_args(dptr, n_elements, precision);
+    launch_sv_zero(reinterpret_cast<void *>(dptr), n_elements, precision, nullptr);
+    check(hipStreamSynchronize(nullptr), "hipStreamSynchronize(zero)");
+  });
+}
+
+int cast_hip_sv_upload(uint64_t dptr, const double *host, size_t n_elements, uint8_t precision,
+                       char *err, size_t len) {
+  return guard(err, len, [&] {
+    check_sv_args(dptr, n_elements, precision);
+    if (!host)
+      throw Error("null host pointer");
+    void *d = reinterpret_cast<void *>(dptr);
+    if (precision == 1) {
+      check(hipMemcpy(d, host, n_elements * sizeof(double), hipMemcpyHostToDevice),
+            "hipMemcpy(HtoD)");
+    } else {
+      std::vector<float> f(n_elements);
+      for (size_t i = 0; i < n_elements; ++i)
+        f[i] = float(host[i]);
+      check(hipMemcpy(d, f.data(), n_elements * sizeof(float), hipMemcpyHostToDevice),
+            "hipMemcpy(HtoD f32)");
+    }
+  });
+}
+
+int cast_hip_sv_download(uint64_t dptr, double *host, size_t n_elements, uint8_t precision,
+                         char *err, size_t len) {
+  return guard(err, len, [&] {
+    check_sv_args(dptr, n_elements, precision);
+    if (!host)
+      throw Error("null host pointer");
+    const void *d = reinterpret_cast<const void *>(dptr);
+    if (precision == 1) {
+      check(hipMemcpy(host, d, n_elements * sizeof(double), hipMemcpyDeviceToHost),
+            "hipMemcpy(DtoH)");
+    } else {
+      std::vector<float> f(n_elements);
+      check(hipMemcpy(f.data(), d, n_elements * sizeof(float), hipMemcpyDeviceToHost),
+            "hipMemcpy(DtoH f32)");
+      for (size_t i = 0; i < n_elements; ++i)
+        host[i] = double(f[i]);
+    }
+  });
+}
+
+int cast_hip_norm_squared(uint64_t dptr, size_t n_elements, uint8_t precision, double *out,
+                          char *err, size_t len) {
+  return guard(err, len, [&] {
+    check_sv_args(dptr, n_elements, precision);
+    if (!out)
+      throw Error("null output pointer");
+    *out = norm_squared(reinterpret_cast<const void *>(dptr), n_elements, precision, nullptr);
+  });
+}
+
+int cast_hip_scale(uint64_t dptr, size_t n_elements, uint8_t precision, double scale, char *err,
+                   size_t len) {
+  return guard(err, len, [&] {
+    check_sv_args(dptr, n_elements, precision);
+    launch_scale(reinterpret_cast<void *>(dptr), n_elements, precision, scale, nullptr);
+    check(hipStreamSynchronize(nullptr), "hipStreamSynchronize(scale)");
+  });
+}
+
+int cast_hip_gate_plan_create(const cast_hip_kernel_gen_spec_t *spec,
+                              const cast_complex64_t *matrix, size_t matrix_len,
+                              const uint32_t *qubits, size_t n_qubits, void **out_plan,
+                              char *err, size_t len) {
+  return guard(err, len, [&] {
+    if (!out_plan)
+      throw Error("out_plan must not be null");
+    *out_plan = nullptr;
+    *out_plan = build_plan(spec, matrix, matrix_len, qubits, n_qubits);
+  });
+}
+
+void cast_hip_gate_plan_destroy(void *plan) { free_plan(static_cast<GatePlan *>(plan)); }
+
+int cast_hip_gate_plan_info(const void *plan, cast_hip_gate_plan_info_t *out, char *err,
+                            size_t len) {
+  return guard(err, len, [&] {
+    if (!plan || !out)
+      throw Error("null argument");
+    const GatePlan *p = static_cast<const GatePlan *>(plan);
+    std::memset(out, 0, sizeof(*out));
+    out->n_qubits = p->k;
+    out->precision = p->precision;
+    out->variant = p->variant;
+    out->nnz_rows = p->variant == CAST_HIP_VARIANT_MONOMIAL ? p->n_active : (1u << p->k);
+    out->scalar_nnz = p->scalar_nnz;
+    for (uint32_t b = 0; b < p->k; ++b)
+      out->qubits[b] = p->q[b];
+  });
+}
+
+int cast_hip_gate_apply(const void *plan, void *stream, uint64_t sv_dptr, uint32_t sv_n_qubits,
+                        char *err, size_t len) {
+  return guard(err, len, [&] {
+    if (!plan)
+      throw Error("null plan");
+    if (!sv_dptr)
+      throw Error("null statevector pointer");
+    launch_gate(*static_cast<const GatePlan *>(plan), static_cast<hipStream_t>(stream),
+                reinterpret_cast<void *>(sv_dptr), sv_n_qubits);
+  });
+}
+
+int cast_hip_gate_apply_batch(const void *const *plans, const uint64_t *svs,
+                              const uint32_t *sv_n_qubits, size_t n_items, void *stream,
+                              char *err, size_t len) {
+  return guard(err, len, [&] {
+    if (n_items == 0)
+      return;
+    if (!plans || !svs || !sv_n_qubits)
+      throw Error("null argument");
+    std::vector<BatchItem> items(n_items);
+    const GatePlan *p0 = static_cast<const GatePlan *>(plans[0]);
+    if (!p0)
+      throw Error("null plan at index 0");
+    for (size_t i = 0; i < n_items; ++i) {
+      const GatePlan *p = static_cast<const GatePlan *>(plans[i]);
+      if (!p)
+        throw Error("null plan at index " + std::to_string(i));
+      if (p->k != p0->k || p->precision != p0->precision)
+        throw Error("batched plans must share k and precision");
+      if (!svs[i])
+        throw Error("null statevector at index " + std::to_string(i));
+      const uint32_t n = sv_n_qubits[i];
+      if (n < p->k || p->q[p->k - 1] >= n || n > 40)
+        throw Error("item " + std::to_string(i) + ": gate does not fit a " +
+                    std::to_string(n) + "-qubit statevector");
+      BatchItem &it = items[i];
+      it.sv = reinterpret_cast<void *>(svs[i]);
+      it.mat = p->d_mat;
+      it.combos = uint64_t(1) << (n - p->k);
+      it.k = uint8_t(p->k);
+      for (uint32_t b = 0; b < 6; ++b)
+        it.q[b] = uint8_t(b < p->k ? p->q[b] : 0);
+    }
+    launch_gate_batch(items.data(), n_items, p0->precision, p0->k,
+                      static_cast<hipStream_t>(stream));
+  });
+}
+
+int cast_hip_expect_zz(uint64_t sv_dptr, uint32_t sv_n_qubits, uint8_t precision,
+                       const uint32_t *pairs, size_t n_pairs, double *out, void *stream,
+                       char *err, size_t len) {
+  return guard(err, len, [&] {
+    check_precision(precision);
+    if (!sv_dptr)
+      throw Error("null statevector pointer");
+    if (n_pairs > 0 && (!pairs || !out))
+      throw Error("null pairs/out");
+    if (sv_n_qubits > 40)
+      throw Error("statevector qubit count out of range");
+    for (size_t p = 0; p < 2 * n_pairs; ++p)
+      if (pairs[p] >= sv_n_qubits)
+        throw Error("pair qubit " + std::to_string(pairs[p]) + " outside the statevector");
+    expect_zz(reinterpret_cast<const void *>(sv_dptr), sv_n_qubits, precision, pairs, n_pairs,
+              out, static_cast<hipStream_t>(stream));
+  });
+}
+
+} // extern "C"

@@ -1,0 +1,110 @@
+// Shared helpers of the gfx950 backend library (libcast_hip.so).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+namespace cast_hip {
+
+// Error carried from the implementation to the C boundary, where it is
+// turned into (return 1, err_buf) — the convention of
+// src/cpp/cuda/ffi_cuda.cpp:1-13 and src/cpp/internal/err_buf.hpp:12-23.
+struct Error : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+inline void write_err(char *buf, size_t len, const std::string &msg) {
+  if (buf == nullptr || len == 0)
+    return;
+  const size_t n = msg.size() < len - 1 ? msg.size() : len - 1;
+  std::memcpy(buf, msg.data(), n);
+  buf[n] = '\0';
+}
+
+inline void clear_err(char *buf, size_t len) {
+  if (buf != nullptr && len > 0)
+    buf[0] = '\0';
+}
+
+inline void check(hipError_t rc, const char *what) {
+  if (rc != hipSuccess)
+    throw Error(std::string(what) + " failed: " + hipGetErrorString(rc));
+}
+
+// Throws unless a HIP device is present; selects nothing (the caller's
+// current device is used, device 0 by default).
+void ensure_device();
+
+// Cached properties of the current device.
+struct DeviceProps {
+  int device = 0;
+  int n_cu = 0;
+  int warp = 64;
+  size_t lds_per_block = 0;
+  int l2_bytes = 0;
+  std::string arch;
+  std::string name;
+};
+const DeviceProps &device_props();
+
+inline size_t scalar_bytes(uint8_t precision) { return precision == 0 ? 4 : 8; }
+
+inline void check_precision(uint8_t precision) {
+  if (precision > 1)
+    throw Error("precision must be 0 (F32) or 1 (F64), got " + std::to_string(precision));
+}
+
+// Geometry of one gate on one statevector.  q[] holds the ascending target
+// qubits; combos = 2^(n-k) is the number of K-amplitude groups.
+struct GateGeom {
+  uint64_t combos;
+  uint32_t q[6];
+  uint32_t k;
+  uint32_t pad;
+};
+
+// Plan built by cast_hip_gate_plan_create.
+struct GatePlan {
+  uint32_t k = 0;
+  uint8_t precision = 1;
+  uint32_t variant = 0;
+  uint32_t q[8] = {};
+  int device = 0;
+  void *d_mat = nullptr;       // K*K complex, interleaved, device precision
+  double *h_mat = nullptr;     // snapped matrix in f64 (host copy, 2*K*K)
+  uint64_t scalar_nnz = 0;
+  // Monomial form: rows that differ from identity, each with one source col.
+  uint32_t n_active = 0;
+  uint8_t act_row[64] = {};
+  uint8_t act_col[64] = {};
+  double act_re[64] = {};
+  double act_im[64] = {};
+};
+
+// Launch entry points implemented in gate_kernels.hip.
+void launch_gate(const GatePlan &plan, hipStream_t stream, void *sv, uint32_t n_qubits);
+struct BatchItem {
+  void *sv;
+  const void *mat;
+  uint64_t combos;
+  uint8_t q[6];
+  uint8_t k;
+  uint8_t pad;
+};
+void launch_gate_batch(const BatchItem *items, size_t n, uint8_t precision, uint32_t k,
+                       hipStream_t stream);
+
+// Statevector utility kernels implemented in sv_kernels.hip.
+void launch_sv_zero(void *sv, size_t n_elements, uint8_t precision, hipStream_t stream);
+double norm_squared(const void *sv, size_t n_elements, uint8_t precision, hipStream_t stream);
+void launch_scale(void *sv, size_t n_elements, uint8_t precision, double scale,
+                  hipStream_t stream);
+void expect_zz(const void *sv, uint32_t n_qubits, uint8_t precision, const uint32_t *pairs,
+               size_t n_pairs, double *out, hipStream_t stream);
+
+} // namespace cast_hip

@@ -1,0 +1,244 @@
+// Statevector utility kernels: |0> fill, f64-accumulated norm^2, scale and
+// <Z_a Z_b> expectations.
+//
+// Counterparts in the reference: cuda_exec.cpp:452-479 (memset + one H->D
+// scalar for |0>), cuda_kernels.cu:123-196 (normSqPass1/2 with f64
+// accumulation, scaleKernel).  Here |0> is one fill kernel (no memset node,
+// no host copy), and every reduction accumulates in f64 whatever the
+// statevector precision.
+
+#include "common.hpp"
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+namespace cast_hip {
+namespace {
+
+constexpr unsigned kRedBlock = 256;
+constexpr unsigned kMaxRedGrid = 2048;
+constexpr unsigned kMaxPairs = 32;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+    v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Block-wide sum; every thread gets the result.
+__device__ __forceinline__ double block_sum(double v, double *lds) {
+  v = wave_sum(v);
+  const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0)
+    lds[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+  const unsigned nw = (blockDim.x + 63u) >> 6;
+  for (unsigned w = 0; w < nw; ++w)
+    t += lds[w];
+  return t;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kRedBlock) k_zero(T *__restrict__ x, size_t n) {
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    x[i] = (i == 0) ? T(1) : T(0);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kRedBlock) k_norm_pass1(const T *__restrict__ x, size_t n,
+                                                          double *__restrict__ partial) {
+  __shared__ double lds[kRedBlock / 64];
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  const size_t tid = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  constexpr int VW = 16 / sizeof(T);
+  using VT = typename std::conditional<sizeof(T) == 8, double2, float4>::type;
+  const size_t nv = n / VW;
+  const VT *xv = reinterpret_cast<const VT *>(x);
+  for (size_t i = tid; i < nv; i += stride) {
+    const VT v = xv[i];
+    const T *e = reinterpret_cast<const T *>(&v);
+#pragma unroll
+    for (int j = 0; j < VW; ++j) {
+      const double d = double(e[j]);
+      acc = __builtin_fma(d, d, acc);
+    }
+  }
+  for (size_t i = nv * VW + tid; i < n; i += stride) {
+    const double d = double(x[i]);
+    acc = __builtin_fma(d, d, acc);
+  }
+  const double s = block_sum(acc, lds);
+  if (threadIdx.x == 0)
+    partial[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(kRedBlock) k_sum_partials(const double *__restrict__ partial,
+                                                            unsigned n, unsigned width,
+                                                            double *__restrict__ out) {
+  // Reduces column `blockIdx.x` of a row-major (n x width) array.
+  __shared__ double lds[kRedBlock / 64];
+  double acc = 0.0;
+  for (unsigned i = threadIdx.x; i < n; i += blockDim.x)
+    acc += partial[size_t(i) * width + blockIdx.x];
+  const double s = block_sum(acc, lds);
+  if (threadIdx.x == 0)
+    out[blockIdx.x] = s;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kRedBlock) k_scale(T *__restrict__ x, size_t n, double scale) {
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    x[i] = T(double(x[i]) * scale);
+}
+
+struct PairArgs {
+  uint32_t n_pairs;
+  uint32_t a[kMaxPairs];
+  uint32_t b[kMaxPairs];
+};
+
+template <typename T>
+__global__ void __launch_bounds__(kRedBlock) k_expect_zz(const T *__restrict__ sv, uint64_t n_amp,
+                                                         PairArgs pa, double *__restrict__ partial) {
+  __shared__ double lds[kRedBlock / 64];
+  using VT = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
+  const VT *amp = reinterpret_cast<const VT *>(sv);
+  double acc[kMaxPairs];
+#pragma unroll
+  for (unsigned p = 0; p < kMaxPairs; ++p)
+    acc[p] = 0.0;
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < n_amp; j += stride) {
+    const VT v = amp[j];
+    const double re = double(v.x), im = double(v.y);
+    const double pr = __builtin_fma(re, re, im * im);
+#pragma unroll
+    for (unsigned p = 0; p < kMaxPairs; ++p)
+      if (p < pa.n_pairs) {
+        const uint64_t par = ((j >> pa.a[p]) ^ (j >> pa.b[p])) & 1u;
+        acc[p] += par ? -pr : pr;
+      }
+  }
+#pragma unroll
+  for (unsigned p = 0; p < kMaxPairs; ++p)
+    if (p < pa.n_pairs) {
+      const double s = block_sum(acc[p], lds);
+      if (threadIdx.x == 0)
+        partial[size_t(blockIdx.x) * pa.n_pairs + p] = s;
+    }
+}
+
+unsigned red_grid(uint64_t work) {
+  const uint64_t need = (work + kRedBlock - 1) / kRedBlock;
+  const uint64_t cap = std::min<uint64_t>(kMaxRedGrid, uint64_t(device_props().n_cu) * 4);
+  return unsigned(std::max<uint64_t>(1, std::min(need, cap)));
+}
+
+// Reduction scratch (device partials + pinned host result), one per process
+// and guarded by a mutex so concurrent callers are serialised.
+struct RedScratch {
+  std::mutex mu;
+  double *d_partial = nullptr; // kMaxRedGrid * kMaxPairs
+  double *d_out = nullptr;     // kMaxPairs
+  double *h_out = nullptr;     // pinned, kMaxPairs
+  void ensure() {
+    if (d_partial)
+      return;
+    check(hipMalloc(&d_partial, sizeof(double) * kMaxRedGrid * kMaxPairs), "hipMalloc(partial)");
+    check(hipMalloc(&d_out, sizeof(double) * kMaxPairs), "hipMalloc(out)");
+    check(hipHostMalloc(&h_out, sizeof(double) * kMaxPairs, hipHostMallocDefault),
+          "hipHostMalloc(out)");
+  }
+};
+RedScratch &scratch() {
+  static RedScratch s;
+  return s;
+}
+
+} // namespace
+
+void launch_sv_zero(void *sv, size_t n_elements, uint8_t precision, hipStream_t stream) {
+  const unsigned grid = red_grid(n_elements) * 4;
+  if (precision == 0)
+    hipLaunchKernelGGL(k_zero<float>, dim3(grid), dim3(kRedBlock), 0, stream,
+                       static_cast<float *>(sv), n_elements);
+  else
+    hipLaunchKernelGGL(k_zero<double>, dim3(grid), dim3(kRedBlock), 0, stream,
+                       static_cast<double *>(sv), n_elements);
+  check(hipGetLastError(), "hipLaunchKernelGGL(k_zero)");
+}
+
+double norm_squared(const void *sv, size_t n_elements, uint8_t precision, hipStream_t stream) {
+  RedScratch &s = scratch();
+  std::lock_guard<std::mutex> lock(s.mu);
+  s.ensure();
+  const unsigned grid = red_grid(n_elements / (precision == 0 ? 4 : 2));
+  if (precision == 0)
+    hipLaunchKernelGGL(k_norm_pass1<float>, dim3(grid), dim3(kRedBlock), 0, stream,
+                       static_cast<const float *>(sv), n_elements, s.d_partial);
+  else
+    hipLaunchKernelGGL(k_norm_pass1<double>, dim3(grid), dim3(kRedBlock), 0, stream,
+                       static_cast<const double *>(sv), n_elements, s.d_partial);
+  check(hipGetLastError(), "hipLaunchKernelGGL(k_norm_pass1)");
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kRedBlock), 0, stream, s.d_partial, grid, 1u,
+                     s.d_out);
+  check(hipGetLastError(), "hipLaunchKernelGGL(k_sum_partials)");
+  check(hipMemcpyAsync(s.h_out, s.d_out, sizeof(double), hipMemcpyDeviceToHost, stream),
+        "hipMemcpyAsync(norm)");
+  check(hipStreamSynchronize(stream), "hipStreamSynchronize(norm)");
+  return s.h_out[0];
+}
+
+void launch_scale(void *sv, size_t n_elements, uint8_t precision, double scale,
+                  hipStream_t stream) {
+  const unsigned grid = red_grid(n_elements) * 4;
+  if (precision == 0)
+    hipLaunchKernelGGL(k_scale<float>, dim3(grid), dim3(kRedBlock), 0, stream,
+                       static_cast<float *>(sv), n_elements, scale);
+  else
+    hipLaunchKernelGGL(k_scale<double>, dim3(grid), dim3(kRedBlock), 0, stream,
+                       static_cast<double *>(sv), n_elements, scale);
+  check(hipGetLastError(), "hipLaunchKernelGGL(k_scale)");
+}
+
+void expect_zz(const void *sv, uint32_t n_qubits, uint8_t precision, const uint32_t *pairs,
+               size_t n_pairs, double *out, hipStream_t stream) {
+  RedScratch &s = scratch();
+  std::lock_guard<std::mutex> lock(s.mu);
+  s.ensure();
+  const uint64_t n_amp = uint64_t(1) << n_qubits;
+  for (size_t off = 0; off < n_pairs; off += kMaxPairs) {
+    PairArgs pa{};
+    pa.n_pairs = uint32_t(std::min<size_t>(kMaxPairs, n_pairs - off));
+    for (uint32_t p = 0; p < pa.n_pairs; ++p) {
+      pa.a[p] = pairs[2 * (off + p)];
+      pa.b[p] = pairs[2 * (off + p) + 1];
+    }
+    const unsigned grid = red_grid(n_amp);
+    if (precision == 0)
+      hipLaunchKernelGGL(k_expect_zz<float>, dim3(grid), dim3(kRedBlock), 0, stream,
+                         static_cast<const float *>(sv), n_amp, pa, s.d_partial);
+    else
+      hipLaunchKernelGGL(k_expect_zz<double>, dim3(grid), dim3(kRedBlock), 0, stream,
+                         static_cast<const double *>(sv), n_amp, pa, s.d_partial);
+    check(hipGetLastError(), "hipLaunchKernelGGL(k_expect_zz)");
+    hipLaunchKernelGGL(k_sum_partials, dim3(pa.n_pairs), dim3(kRedBlock), 0, stream, s.d_partial,
+                       grid, pa.n_pairs, s.d_out);
+    check(hipGetLastError(), "hipLaunchKernelGGL(k_sum_partials)");
+    check(hipMemcpyAsync(s.h_out, s.d_out, sizeof(double) * pa.n_pairs, hipMemcpyDeviceToHost,
+                         stream),
+          "hipMemcpyAsync(zz)");
+    check(hipStreamSynchronize(stream), "hipStreamSynchronize(zz)");
+    for (uint32_t p = 0; p < pa.n_pairs; ++p)
+      out[off + p] = s.h_out[p];
+  }
+}
+
+} // namespace cast_hip

@@ -1,0 +1,412 @@
+// Python bindings of the C++ host layer (module cast_simulator_amd._host).
+// Names follow the reference's Rust API (types, CircuitGraph, fusion,
+// cost_model, profile, cuda::{CudaKernelManager, CudaStatevector} ->
+// Hip*, simulator::Simulator).
+#include <pybind11/complex.h>
+#include <pybind11/functional.h>
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include "circuit.hpp"
+#include "cost_model.hpp"
+#include "fusion.hpp"
+#include "hip_backend.hpp"
+#include "profile.hpp"
+#include "types.hpp"
+
+namespace py = pybind11;
+using namespace cast;
+using namespace cast::hip;
+
+namespace {
+
+py::array_t<cplx> to_numpy(const ComplexSquareMatrix &m) {
+  const ssize_t n = ssize_t(m.edge_size());
+  py::array_t<cplx> a({n, n});
+  auto r = a.mutable_unchecked<2>();
+  for (ssize_t i = 0; i < n; ++i)
+    for (ssize_t j = 0; j < n; ++j)
+      r(i, j) = m.get(size_t(i), size_t(j));
+  return a;
+}
+
+ComplexSquareMatrix from_numpy(py::array_t<cplx, py::array::c_style | py::array::forcecast> a) {
+  if (a.ndim() != 2 || a.shape(0) != a.shape(1))
+    throw Error("matrix must be a square 2-D array");
+  const size_t n = size_t(a.shape(0));
+  auto r = a.unchecked<2>();
+  ComplexSquareMatrix m(n);
+  for (size_t i = 0; i < n; ++i)
+    for (size_t j = 0; j < n; ++j)
+      m.set(i, j, r(ssize_t(i), ssize_t(j)));
+  return m;
+}
+
+py::array_t<double> vec_to_numpy(const std::vector<double> &v) {
+  py::array_t<double> a(ssize_t(v.size()));
+  std::copy(v.begin(), v.end(), a.mutable_data());
+  return a;
+}
+
+std::vector<double> numpy_to_vec(py::array_t<double, py::array::c_style | py::array::forcecast> a) {
+  return std::vector<double>(a.data(), a.data() + a.size());
+}
+
+std::vector<SweepEntry> entries_from(const std::vector<std::tuple<double, double, double>> &t) {
+  std::vector<SweepEntry> v;
+  for (const auto &e : t)
+    v.push_back({std::get<0>(e), std::get<1>(e), std::get<2>(e)});
+  return v;
+}
+
+} // namespace
+
+PYBIND11_MODULE(_host, m) {
+  m.doc() = "CAST MI355X host layer: gate types, circuit graph, fusion, cost model, "
+            "profiler and the gfx950 kernel manager / simulator.";
+  py::register_exception<Error>(m, "CastError", PyExc_RuntimeError);
+
+  // ── types ──────────────────────────────────────────────────────────────
+  py::class_<QuantumGate, std::shared_ptr<QuantumGate>>(m, "QuantumGate")
+      .def(py::init([](py::array_t<cplx> mat, std::vector<uint32_t> qubits) {
+             return QuantumGate(from_numpy(mat), std::move(qubits));
+           }),
+           py::arg("matrix"), py::arg("qubits"))
+      .def_static("from_reals", &QuantumGate::from_reals)
+      .def_static("x", &QuantumGate::x)
+      .def_static("y", &QuantumGate::y)
+      .def_static("z", &QuantumGate::z)
+      .def_static("h", &QuantumGate::h)
+      .def_static("s", &QuantumGate::s)
+      .def_static("t", &QuantumGate::t)
+      .def_static("rx", &QuantumGate::rx)
+      .def_static("ry", &QuantumGate::ry)
+      .def_static("rz", &QuantumGate::rz)
+      .def_static("u3", &QuantumGate::u3)
+      .def_static("cx", &QuantumGate::cx)
+      .def_static("cz", &QuantumGate::cz)
+      .def_static("cp", &QuantumGate::cp)
+      .def_static("swap", &QuantumGate::swap)
+      .def_static("ccx", &QuantumGate::ccx)
+      .def_static("depolarizing", &QuantumGate::depolarizing)
+      .def_static("random_unitary",
+                  [](std::vector<uint32_t> q, uint64_t seed) {
+                    Rng rng(seed);
+                    return QuantumGate::random_unitary(q, rng);
+                  })
+      .def_static("random_sparse",
+                  [](std::vector<uint32_t> q, double sparsity, uint64_t seed) {
+                    Rng rng(seed);
+                    return QuantumGate::random_sparse(q, sparsity, rng);
+                  })
+      .def_static("random_arithmetic_intensity",
+                  [](uint32_t n_sv, double ai, uint64_t seed) {
+                    Rng rng(seed);
+                    return QuantumGate::random_arithmetic_intensity(n_sv, ai, rng);
+                  })
+      .def("matrix", [](const QuantumGate &g) { return to_numpy(g.matrix()); })
+      .def("qubits", &QuantumGate::qubits)
+      .def("n_qubits", &QuantumGate::n_qubits)
+      .def("effective_n_qubits", &QuantumGate::effective_n_qubits)
+      .def("is_unitary", &QuantumGate::is_unitary)
+      .def("noise_branches",
+           [](const QuantumGate &g) {
+             py::list out;
+             if (g.noise_model())
+               for (const auto &b : g.noise_model()->branches)
+                 out.append(py::make_tuple(b.first, to_numpy(b.second)));
+             return out;
+           })
+      .def("with_noise",
+           [](const QuantumGate &g, const std::vector<std::pair<double, py::array_t<cplx>>> &ops) {
+             std::vector<std::pair<double, ComplexSquareMatrix>> v;
+             for (const auto &o : ops)
+               v.emplace_back(o.first, from_numpy(o.second));
+             return g.with_noise(v);
+           })
+      .def("with_depolarizing", &QuantumGate::with_depolarizing)
+      .def("scalar_nnz", &QuantumGate::scalar_nnz)
+      .def("arithmetic_intensity", &QuantumGate::arithmetic_intensity)
+      .def("matmul", &QuantumGate::matmul)
+      .def("to_density_matrix_gate", &QuantumGate::to_density_matrix_gate)
+      .def("__eq__", &QuantumGate::operator==)
+      .def("__repr__", [](const QuantumGate &g) {
+        std::string s = "QuantumGate(k=" + std::to_string(g.n_qubits()) + ", qubits=[";
+        for (size_t i = 0; i < g.qubits().size(); ++i)
+          s += (i ? "," : "") + std::to_string(g.qubits()[i]);
+        return s + (g.is_unitary() ? "])" : "], noisy)");
+      });
+
+  m.def("expand_matrix", [](py::array_t<cplx> mat, std::vector<uint32_t> gq,
+                            std::vector<uint32_t> uq) {
+    return to_numpy(expand_matrix(from_numpy(mat), gq, uq));
+  });
+  m.def("superoperator_of", [](py::array_t<cplx> mat) {
+    return to_numpy(superoperator_of(from_numpy(mat)));
+  });
+  m.def("haar_su2", [](uint64_t seed) {
+    Rng rng(seed);
+    return to_numpy(haar_su2(rng));
+  });
+  m.def("random_unitary_matrix", [](size_t edge, uint64_t seed) {
+    Rng rng(seed);
+    return to_numpy(ComplexSquareMatrix::random_unitary(edge, rng));
+  });
+  m.def("random_statevector", [](uint32_t n, uint64_t seed) {
+    return vec_to_numpy(random_statevector(n, seed));
+  });
+
+  py::class_<QuantumCircuit>(m, "QuantumCircuit")
+      .def(py::init<uint32_t>())
+      .def("add", [](QuantumCircuit &c, const QuantumGate &g) { c.add(g); })
+      .def("gates",
+           [](const QuantumCircuit &c) {
+             std::vector<std::shared_ptr<QuantumGate>> out;
+             for (const auto &g : c.gates())
+               out.push_back(std::make_shared<QuantumGate>(*g));
+             return out;
+           })
+      .def("n_qubits", &QuantumCircuit::n_qubits)
+      .def("__len__", &QuantumCircuit::len)
+      .def_static("qft", &QuantumCircuit::qft, py::arg("n"), py::arg("swap") = true);
+  m.def("random_layer_circuit", &random_layer_circuit, py::arg("n"), py::arg("depth"),
+        py::arg("seed"));
+
+  py::class_<CircuitGraph>(m, "CircuitGraph")
+      .def(py::init<>())
+      .def_static("from_circuit", &CircuitGraph::from_circuit)
+      .def("n_qubits", &CircuitGraph::n_qubits)
+      .def("n_rows", &CircuitGraph::n_rows)
+      .def("insert_gate",
+           [](CircuitGraph &g, const QuantumGate &q) {
+             return g.insert_gate(std::make_shared<QuantumGate>(q));
+           })
+      .def("gates_in_row_order",
+           [](const CircuitGraph &g) {
+             std::vector<std::shared_ptr<QuantumGate>> out;
+             for (const auto &p : g.gates_in_row_order())
+               out.push_back(std::make_shared<QuantumGate>(*p));
+             return out;
+           })
+      .def("row_gate_ids", &CircuitGraph::row_gate_ids)
+      .def("gate_id_at",
+           [](const CircuitGraph &g, size_t r, size_t q) -> py::object {
+             const GateId id = g.gate_id_at(r, q);
+             if (id == kNoGate)
+               return py::none();
+             return py::int_(id);
+           })
+      .def("check_consistency", &CircuitGraph::check_consistency)
+      .def("squeeze", &CircuitGraph::squeeze)
+      .def("copy", [](const CircuitGraph &g) { return CircuitGraph(g); });
+
+  // ── cost model / fusion ────────────────────────────────────────────────
+  py::class_<CostModel, std::shared_ptr<CostModel>>(m, "CostModel")
+      .def("cost_of", &CostModel::cost_of);
+  py::class_<SizeOnlyCostModel, CostModel, std::shared_ptr<SizeOnlyCostModel>>(m,
+                                                                               "SizeOnlyCostModel")
+      .def(py::init<size_t, size_t, double, bool>(), py::arg("max_size"),
+           py::arg("max_ai") = ~size_t(0), py::arg("zero_tol") = 0.0,
+           py::arg("trajectory_mode") = false);
+
+  py::class_<ProfileConfig>(m, "ProfileConfig")
+      .def(py::init<>())
+      .def_readwrite("backend", &ProfileConfig::backend)
+      .def_readwrite("name", &ProfileConfig::name)
+      .def_readwrite("arch", &ProfileConfig::arch)
+      .def_readwrite("n_threads", &ProfileConfig::n_threads)
+      .def_readwrite("precision", &ProfileConfig::precision)
+      .def_readwrite("n_qubits", &ProfileConfig::n_qubits);
+
+  py::class_<HardwareProfile>(m, "HardwareProfile")
+      .def(py::init<>())
+      .def_static("from_roofline", &HardwareProfile::from_roofline)
+      .def_static("from_measurements", &HardwareProfile::from_measurements)
+      .def_static("from_json", &HardwareProfile::from_json)
+      .def_static("load", &HardwareProfile::load)
+      .def("to_json", &HardwareProfile::to_json)
+      .def("save", &HardwareProfile::save)
+      .def_readwrite("config", &HardwareProfile::config)
+      .def_readwrite("peak_bw_gib_s", &HardwareProfile::peak_bw_gib_s)
+      .def_readwrite("peak_gflops_s", &HardwareProfile::peak_gflops_s)
+      .def_readwrite("bw_slope", &HardwareProfile::bw_slope)
+      .def_readwrite("crossover_ai", &HardwareProfile::crossover_ai)
+      .def_property_readonly("raw", [](const HardwareProfile &p) {
+        py::list out;
+        for (const auto &e : p.raw)
+          out.append(py::make_tuple(e.ai, e.gflops_s, e.gib_s));
+        return out;
+      });
+  m.def("compute_bw_slope", &compute_bw_slope);
+
+  py::class_<HardwareAdaptiveCostModel, CostModel, std::shared_ptr<HardwareAdaptiveCostModel>>(
+      m, "HardwareAdaptiveCostModel")
+      .def(py::init<const HardwareProfile &, size_t, bool, double>(), py::arg("profile"),
+           py::arg("max_size"), py::arg("trajectory_mode") = false, py::arg("zero_tol") = 1e-12)
+      .def_readonly("crossover_ai", &HardwareAdaptiveCostModel::crossover_ai);
+
+  py::class_<FusionConfig>(m, "FusionConfig")
+      .def_static("size_only", &FusionConfig::size_only)
+      .def_static("size_only_trajectory", &FusionConfig::size_only_trajectory)
+      .def_static("aggressive", &FusionConfig::aggressive)
+      .def_static("hardware_adaptive", &FusionConfig::hardware_adaptive)
+      .def_static("hardware_adaptive_trajectory", &FusionConfig::hardware_adaptive_trajectory)
+      .def_readwrite("size_max", &FusionConfig::size_max)
+      .def_readwrite("benefit_margin", &FusionConfig::benefit_margin)
+      .def_readwrite("zero_tol", &FusionConfig::zero_tol)
+      .def("cost_of", [](const FusionConfig &c, const QuantumGate &g) {
+        return c.cost_model->cost_of(g);
+      });
+
+  py::class_<FusionDecision>(m, "FusionDecision")
+      .def_readonly("cdd_size", &FusionDecision::cdd_size)
+      .def_readonly("n_candidates", &FusionDecision::n_candidates)
+      .def_readonly("candidates", &FusionDecision::candidates)
+      .def_readonly("product_n_qubits", &FusionDecision::product_n_qubits)
+      .def_readonly("product_ai", &FusionDecision::product_ai)
+      .def_readonly("old_cost", &FusionDecision::old_cost)
+      .def_readonly("new_cost", &FusionDecision::new_cost)
+      .def_readonly("benefit", &FusionDecision::benefit)
+      .def_readonly("accepted", &FusionDecision::accepted);
+  py::class_<FusionLog>(m, "FusionLog")
+      .def_readonly("decisions", &FusionLog::decisions)
+      .def("n_accepted", &FusionLog::n_accepted)
+      .def("n_rejected", &FusionLog::n_rejected)
+      .def("summary", &FusionLog::summary);
+
+  m.def("apply_size_two_fusion", &apply_size_two_fusion);
+  m.def("optimize", &optimize);
+  m.def("optimize_with_log", &optimize_with_log);
+
+  // ── profile / timing ───────────────────────────────────────────────────
+  m.def("fit_roofline", [](const std::vector<std::tuple<double, double, double>> &s) {
+    auto v = entries_from(s);
+    std::sort(v.begin(), v.end(), [](auto &a, auto &b) { return a.ai < b.ai; });
+    const auto p = fit_roofline(v);
+    return py::make_tuple(p.crossover_ai, p.bw_slope, roofline_r2(v, p));
+  });
+  m.def("fit_from_samples",
+        [](ProfileConfig c, const std::vector<std::tuple<double, double, double>> &s) {
+          return fit_from_samples(std::move(c), entries_from(s));
+        });
+  m.def("adaptive_sweep",
+        [](std::function<std::tuple<double, double, double>(double, double)> probe,
+           ProfileConfig c, double budget) {
+          return adaptive_sweep(
+              [&](double ai, double b) {
+                const auto t = probe(ai, b);
+                return SweepEntry{std::get<0>(t), std::get<1>(t), std::get<2>(t)};
+              },
+              std::move(c), budget);
+        });
+
+  // ── gfx950 backend ─────────────────────────────────────────────────────
+  py::enum_<cast_precision_t>(m, "Precision")
+      .value("F32", CAST_PRECISION_F32)
+      .value("F64", CAST_PRECISION_F64);
+
+  py::class_<HipKernelGenSpec>(m, "HipKernelGenSpec")
+      .def(py::init<>())
+      .def_static("f32", &HipKernelGenSpec::f32)
+      .def_static("f64", &HipKernelGenSpec::f64)
+      .def_readwrite("precision", &HipKernelGenSpec::precision)
+      .def_readwrite("ztol", &HipKernelGenSpec::ztol)
+      .def_readwrite("otol", &HipKernelGenSpec::otol)
+      .def_readwrite("variant", &HipKernelGenSpec::variant);
+
+  py::class_<HipStatevector>(m, "HipStatevector")
+      .def(py::init<uint32_t, cast_precision_t>())
+      .def("n_qubits", &HipStatevector::n_qubits)
+      .def("precision", &HipStatevector::precision)
+      .def("__len__", &HipStatevector::len)
+      .def("dptr", &HipStatevector::dptr)
+      .def("zero", &HipStatevector::zero)
+      .def("upload", [](HipStatevector &s, py::array_t<double> a) { s.upload(numpy_to_vec(a)); })
+      .def("download", [](const HipStatevector &s) { return vec_to_numpy(s.download()); })
+      .def("norm_squared", &HipStatevector::norm_squared)
+      .def("normalize", &HipStatevector::normalize)
+      .def("clone_device", &HipStatevector::clone_device)
+      .def("expect_zz", &HipStatevector::expect_zz);
+
+  py::class_<PlanInfo>(m, "PlanInfo")
+      .def_readonly("k", &PlanInfo::k)
+      .def_readonly("precision", &PlanInfo::precision)
+      .def_readonly("variant", &PlanInfo::variant)
+      .def_readonly("nnz_rows", &PlanInfo::nnz_rows)
+      .def_readonly("scalar_nnz", &PlanInfo::scalar_nnz)
+      .def_readonly("qubits", &PlanInfo::qubits);
+
+  py::class_<HipKernelManager>(m, "HipKernelManager")
+      .def(py::init<HipKernelGenSpec>())
+      .def("generate", &HipKernelManager::generate)
+      .def("apply", &HipKernelManager::apply)
+      .def("sync",
+           [](HipKernelManager &mgr, bool timed) {
+             SyncStats st;
+             {
+               py::gil_scoped_release rel;
+               st = mgr.sync(timed);
+             }
+             py::list ks;
+             for (const auto &k : st.kernels)
+               ks.append(py::make_tuple(k.id, k.gpu_s));
+             return py::make_tuple(ks, st.wall_s);
+           },
+           py::arg("timed") = true)
+      .def("time_adaptive",
+           [](HipKernelManager &mgr, KernelId id, HipStatevector &sv, double budget) {
+             const auto t = mgr.time_adaptive(id, sv, budget);
+             return py::dict(py::arg("n_iters") = t.n_iters, py::arg("mean_s") = t.mean_s,
+                             py::arg("stddev_s") = t.stddev_s, py::arg("min_s") = t.min_s,
+                             py::arg("max_s") = t.max_s);
+           })
+      .def("plan_info", &HipKernelManager::plan_info)
+      .def("n_kernels", &HipKernelManager::n_kernels);
+
+  py::class_<Simulator>(m, "Simulator")
+      .def(py::init<HipKernelGenSpec>())
+      .def_static("f32", &Simulator::f32)
+      .def_static("f64", &Simulator::f64)
+      .def("compile", &Simulator::compile)
+      .def("run",
+           [](Simulator &s, const CircuitGraph &g, HipStatevector &sv) {
+             const auto st = s.run(g, sv);
+             py::list ks;
+             for (const auto &k : st.kernels)
+               ks.append(py::make_tuple(k.id, k.gpu_s));
+             return py::make_tuple(ks, st.wall_s);
+           })
+      .def("simulate", &Simulator::simulate)
+      .def("bench",
+           [](Simulator &s, const CircuitGraph &g, double budget) {
+             const auto r = s.bench(g, budget);
+             return py::dict(py::arg("compile_s") = r.compile_s,
+                             py::arg("samples_s") = r.samples_s, py::arg("n_gates") = r.n_gates,
+                             py::arg("mean_s") = r.exec.mean_s, py::arg("min_s") = r.exec.min_s);
+           })
+      .def("manager", &Simulator::manager, py::return_value_policy::reference_internal);
+
+  m.def("measure_hip",
+        [](HipKernelGenSpec spec, uint32_t n_qubits, double budget_s, uint64_t seed) {
+          HipKernelManager mgr(spec);
+          HipStatevector sv(n_qubits, spec.precision);
+          sv.zero();
+          ProfileConfig c;
+          c.backend = "hip";
+          cast_hip_device_info_t di{};
+          char err[1024];
+          if (cast_hip_device_info(&di, err, sizeof(err)) == 0) {
+            c.name = di.name;
+            c.arch = di.arch;
+          }
+          c.precision = spec.precision == CAST_PRECISION_F32 ? "f32" : "f64";
+          c.n_qubits = n_qubits;
+          return measure(c, n_qubits, spec.scalar_bytes(), budget_s, seed,
+                         [&](const QuantumGate &g, double b) {
+                           const KernelId id = mgr.generate(g);
+                           return mgr.time_adaptive(id, sv, b).mean_s;
+                         });
+        },
+        py::arg("spec"), py::arg("n_qubits"), py::arg("budget_s"), py::arg("seed") = 1);
+}

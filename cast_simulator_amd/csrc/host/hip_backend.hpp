@@ -1,0 +1,158 @@
+// Host-side gfx950 backend: statevector handle, kernel manager and the
+// simulator driver, all built on the C ABI of include/cast_hip.h.
+//
+// Mirrors the reference's Rust CUDA backend:
+//   HipStatevector   <- src/cuda/statevector.rs:44-253 (alloc/zero/upload/
+//                       download/norm_squared/normalize/clone_device)
+//   HipKernelManager <- src/cuda/kernel.rs:705-1253: generate() builds (or
+//                       reuses, keyed on matrix bytes + qubits, :213-232) a
+//                       gate plan; apply() validates and queues (:894-924);
+//                       sync() launches the queue on the manager's stream with
+//                       events between launches and waits (:935-977), returning
+//                       per-kernel GPU times.  There are no modules to cache:
+//                       a plan is ready to launch once built.
+//   Simulator        <- src/simulator/mod.rs:523-547 (simulate) and
+//                       src/simulator/bench.rs:136-167 (bench; a sample is
+//                       the sum of per-kernel GPU event times, mod.rs:295-307).
+#pragma once
+
+#include "cast_hip.h"
+#include "circuit.hpp"
+#include "profile.hpp"
+
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace cast::hip {
+
+struct HipKernelGenSpec {
+  cast_precision_t precision = CAST_PRECISION_F64;
+  double ztol = 1e-12;
+  double otol = 1e-12;
+  uint32_t variant = CAST_HIP_VARIANT_AUTO;
+  // src/cuda/types.rs:31-56 defaults.
+  static HipKernelGenSpec f32() { return {CAST_PRECISION_F32, 1e-6, 1e-6, 0}; }
+  static HipKernelGenSpec f64() { return {CAST_PRECISION_F64, 1e-12, 1e-12, 0}; }
+  size_t scalar_bytes() const { return precision == CAST_PRECISION_F32 ? 4 : 8; }
+};
+
+// Throws cast::Error with the library's message when a C-ABI call fails.
+void throw_if(int rc, const char *err);
+
+class HipStatevector {
+public:
+  HipStatevector(uint32_t n_qubits, cast_precision_t precision);
+  ~HipStatevector();
+  HipStatevector(const HipStatevector &) = delete;
+  HipStatevector &operator=(const HipStatevector &) = delete;
+  HipStatevector(HipStatevector &&o) noexcept;
+
+  uint32_t n_qubits() const { return n_; }
+  cast_precision_t precision() const { return prec_; }
+  size_t len() const { return size_t(1) << n_; }
+  size_t n_elements() const { return n_elements_; }
+  uint64_t dptr() const { return dptr_; }
+
+  void zero();
+  void upload(const std::vector<double> &interleaved);
+  std::vector<double> download() const;
+  double norm_squared() const;
+  void normalize();
+  HipStatevector clone_device() const;
+  std::vector<double> expect_zz(const std::vector<uint32_t> &pairs) const;
+
+private:
+  uint64_t dptr_ = 0;
+  uint32_t n_ = 0;
+  cast_precision_t prec_ = CAST_PRECISION_F64;
+  size_t n_elements_ = 0;
+};
+
+using KernelId = uint64_t;
+
+struct KernelExecTime {
+  KernelId id = 0;
+  double gpu_s = 0.0;
+};
+
+struct SyncStats {
+  std::vector<KernelExecTime> kernels;
+  double wall_s = 0.0;
+};
+
+struct PlanInfo {
+  uint32_t k = 0, precision = 0, variant = 0, nnz_rows = 0;
+  uint64_t scalar_nnz = 0;
+  std::vector<uint32_t> qubits;
+};
+
+class HipKernelManager {
+public:
+  explicit HipKernelManager(HipKernelGenSpec spec);
+  ~HipKernelManager();
+  HipKernelManager(const HipKernelManager &) = delete;
+  HipKernelManager &operator=(const HipKernelManager &) = delete;
+
+  const HipKernelGenSpec &spec() const { return spec_; }
+  KernelId generate(const QuantumGate &gate);
+  void apply(KernelId id, HipStatevector &sv);
+  SyncStats sync(bool timed = true);
+  TimingStats time_adaptive(KernelId id, HipStatevector &sv, double budget_s);
+  PlanInfo plan_info(KernelId id) const;
+  size_t n_kernels() const;
+  void *stream(); // created on first use
+  const void *plan_handle(KernelId id) const;
+
+private:
+  struct Pending {
+    KernelId id;
+    uint64_t dptr;
+    uint32_t n_qubits;
+  };
+  void ensure_stream_locked();
+  void ensure_events_locked(size_t n);
+
+  HipKernelGenSpec spec_;
+  mutable std::mutex mu_;
+  std::unordered_map<KernelId, void *> plans_;
+  std::unordered_map<std::string, KernelId> dedup_;
+  KernelId next_id_ = 0;
+  std::vector<Pending> pending_;
+  void *stream_ = nullptr;
+  std::vector<void *> events_;
+};
+
+// Simulator<Hip>: state-vector simulation of a circuit graph.
+class Simulator {
+public:
+  explicit Simulator(HipKernelGenSpec spec) : spec_(spec), mgr_(spec) {}
+  static Simulator f32() { return Simulator(HipKernelGenSpec::f32()); }
+  static Simulator f64() { return Simulator(HipKernelGenSpec::f64()); }
+
+  HipKernelManager &manager() { return mgr_; }
+  const HipKernelGenSpec &spec() const { return spec_; }
+
+  // Gates of the graph in row order; noisy gates are rejected
+  // (simulator/mod.rs:586-603).
+  std::vector<KernelId> compile(const CircuitGraph &graph);
+  // Applies every gate of `graph` to `sv` and waits.
+  SyncStats run(const CircuitGraph &graph, HipStatevector &sv);
+  // New |0...0> statevector of graph.n_qubits(), simulated.
+  HipStatevector simulate(const CircuitGraph &graph);
+
+  struct BenchResult {
+    double compile_s = 0.0;
+    std::vector<double> samples_s; // per exec: sum of kernel GPU times
+    TimingStats exec;
+    size_t n_gates = 0;
+  };
+  BenchResult bench(const CircuitGraph &graph, double budget_s);
+
+private:
+  HipKernelGenSpec spec_;
+  HipKernelManager mgr_;
+};
+
+} // namespace cast::hip

@@ -1,0 +1,201 @@
+#ifndef CAST_HIP_H
+#define CAST_HIP_H
+
+/*
+ * cast_hip.h — C ABI of the MI355X (gfx950) gate-application backend.
+ *
+ * This header is the drop-in boundary that replaces the reference's CUDA
+ * bridge (src/cpp/include/ffi_cuda.h + ffi_types.h) for the
+ * "apply a fused k-qubit complex matrix to a statevector" path.  A Rust
+ * binding (the equivalent of src/ffi/cuda.rs) binds exactly these symbols.
+ *
+ * Conventions (kept from the reference, src/cpp/cuda/ffi_cuda.cpp:1-13):
+ *   - int-returning calls return 0 on success and 1 on failure;
+ *     constructors return a null handle (or 0 device pointer) on failure.
+ *   - Every call that can fail takes a caller-owned (err_buf, err_buf_len).
+ *     On failure a NUL-terminated, truncated message is written there;
+ *     on success err_buf[0] is set to '\0' (src/cpp/internal/err_buf.hpp:12-23).
+ *   - No C++ exception ever crosses this boundary.
+ *   - Device memory is a raw device address carried as uint64_t; streams,
+ *     events and gate plans are opaque void* handles.
+ *   - Precision is a uint8_t: 0 = F32, 1 = F64 (same values as
+ *     cast_precision_t).
+ *   - n_elements always counts SCALARS: a statevector of n qubits has
+ *     n_elements = 2 * 2^n (interleaved re, im), as in
+ *     src/cuda/statevector.rs:77.
+ *   - The host side of sv_upload / sv_download is always f64 pairs; an F32
+ *     statevector is narrowed / widened inside the library
+ *     (src/cpp/cuda/cuda_exec.cpp:481-533).
+ *
+ * Replaced reference machinery: the per-gate LLVM IR -> PTX -> cubin ->
+ * CUmodule -> CUfunction chain (cast_cuda_compile_gate_ptx,
+ * cast_cuda_ptx_to_cubin, cast_cuda_cubin_load, cast_cuda_module_get_function,
+ * ffi_cuda.h:27-62) becomes ONE host-side "gate plan": the matrix is
+ * classified (zero / +-1 snapping with the reference's ztol/K, otol/K rule),
+ * converted to device precision, uploaded once, and a pre-compiled gfx950
+ * kernel variant is chosen.  No code generation happens at run time.
+ */
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ── Shared types (src/cpp/include/ffi_types.h:13-21) ─────────────────────── */
+
+typedef enum cast_precision_t {
+  CAST_PRECISION_F32 = 0,
+  CAST_PRECISION_F64 = 1,
+} cast_precision_t;
+
+typedef struct cast_complex64_t {
+  double re;
+  double im;
+} cast_complex64_t;
+
+/* ── Gate-plan spec (replaces cast_cuda_kernel_gen_spec_t, ffi_cuda.h:16-23) ─
+ * ztol / otol keep the reference meaning (src/cuda/types.rs:12-56): a matrix
+ * scalar with |v| < ztol/K becomes exactly 0 and one with |v -+ 1| < otol/K
+ * becomes exactly +-1 (src/cpp/cuda/cuda_gen.cpp:67-112).  sm_major/sm_minor
+ * and maxnreg have no meaning on gfx950; `variant` may force a kernel family
+ * (CAST_HIP_VARIANT_*), 0 lets the plan builder choose. */
+typedef struct cast_hip_kernel_gen_spec_t {
+  cast_precision_t precision;
+  double ztol;
+  double otol;
+  uint32_t variant;
+  uint32_t reserved;
+} cast_hip_kernel_gen_spec_t;
+
+#define CAST_HIP_VARIANT_AUTO 0u
+#define CAST_HIP_VARIANT_DENSE 1u    /* register-resident dense matvec, one combo per lane      */
+#define CAST_HIP_VARIANT_MONOMIAL 2u /* <=1 nonzero per row (diag / permutation with phases)    */
+#define CAST_HIP_VARIANT_LDS 3u      /* low-order targets: coalesced tile staged through LDS    */
+#define CAST_HIP_VARIANT_MFMA 4u     /* dense k>=4 contraction on matrix cores                  */
+
+/* Read-only description of a built plan (no reference counterpart: the
+ * reference exposes this through PTX text, src/cuda/kernel.rs:847-882). */
+typedef struct cast_hip_gate_plan_info_t {
+  uint32_t n_qubits;   /* k                                              */
+  uint32_t precision;  /* 0 = F32, 1 = F64                               */
+  uint32_t variant;    /* CAST_HIP_VARIANT_* actually chosen             */
+  uint32_t nnz_rows;   /* rows that differ from the identity row         */
+  uint64_t scalar_nnz; /* nonzero re/im scalars after snapping           */
+  uint32_t qubits[8];  /* ascending target qubits                        */
+} cast_hip_gate_plan_info_t;
+
+/* Device description (replaces cast_cuda_device_sm, ffi_cuda.h:36). */
+typedef struct cast_hip_device_info_t {
+  char name[128];
+  char arch[64];          /* e.g. "gfx950:sramecc+:xnack-"            */
+  uint32_t n_cu;
+  uint32_t wavefront_size;
+  uint32_t lds_per_block; /* bytes                                     */
+  uint32_t l2_bytes;
+  uint32_t clock_khz;
+  uint32_t mem_clock_khz;
+  uint32_t mem_bus_width; /* bits                                      */
+  uint32_t reserved;
+  uint64_t total_mem;     /* bytes                                     */
+} cast_hip_device_info_t;
+
+/* ── Device queries (ffi_cuda.h:36-39) ───────────────────────────────────── */
+
+int cast_hip_device_info(cast_hip_device_info_t *out, char *err_buf, size_t err_buf_len);
+/* Replaces cast_cuda_free_memory (ffi_cuda.h:38, cuda_exec.cpp:133-153). */
+int cast_hip_free_memory(uint64_t *out_free_bytes, uint64_t *out_total_bytes, char *err_buf,
+                         size_t err_buf_len);
+
+/* ── Streams (ffi_cuda.h:43-45) ──────────────────────────────────────────── */
+
+void *cast_hip_stream_create(char *err_buf, size_t err_buf_len);
+void cast_hip_stream_destroy(void *stream);
+int cast_hip_stream_sync(void *stream, char *err_buf, size_t err_buf_len);
+
+/* ── Events (ffi_cuda.h:66-71) ───────────────────────────────────────────── */
+
+void *cast_hip_event_create(char *err_buf, size_t err_buf_len);
+int cast_hip_event_record(void *event, void *stream, char *err_buf, size_t err_buf_len);
+int cast_hip_event_synchronize(void *event, char *err_buf, size_t err_buf_len);
+void cast_hip_event_destroy(void *event);
+int cast_hip_event_elapsed_ms(void *start_event, void *end_event, float *out_ms, char *err_buf,
+                              size_t err_buf_len);
+
+/* ── Device-to-device async copy (ffi_cuda.h:75-76) ─────────────────────── */
+
+int cast_hip_memcpy_dtod_async(uint64_t dst, uint64_t src, size_t n_bytes, void *stream,
+                               char *err_buf, size_t err_buf_len);
+
+/* ── Statevector memory (ffi_cuda.h:80-88) ───────────────────────────────── */
+
+/* Returns 0 on failure (e.g. out of device memory). */
+uint64_t cast_hip_sv_alloc(size_t n_elements, uint8_t precision, char *err_buf,
+                           size_t err_buf_len);
+void cast_hip_sv_free(uint64_t dptr);
+/* Sets |0...0>: every scalar 0, re(amp[0]) = 1 (cuda_exec.cpp:452-479).
+ * Runs on the null stream and returns after completion. */
+int cast_hip_sv_zero(uint64_t dptr, size_t n_elements, uint8_t precision, char *err_buf,
+                     size_t err_buf_len);
+int cast_hip_sv_upload(uint64_t dptr, const double *host_data, size_t n_elements,
+                       uint8_t precision, char *err_buf, size_t err_buf_len);
+int cast_hip_sv_download(uint64_t dptr, double *host_data, size_t n_elements, uint8_t precision,
+                         char *err_buf, size_t err_buf_len);
+
+/* ── Reductions (ffi_cuda.h:94-99; cuda_kernels.cu:123-196) ─────────────── */
+
+/* Sum of squares of all n_elements scalars, accumulated in f64 for both
+ * precisions.  Synchronous (null stream). */
+int cast_hip_norm_squared(uint64_t dptr, size_t n_elements, uint8_t precision,
+                          double *out_norm_sq, char *err_buf, size_t err_buf_len);
+/* x *= scale for all n_elements scalars (scale applied in f64, rounded once).
+ * Synchronous (null stream). */
+int cast_hip_scale(uint64_t dptr, size_t n_elements, uint8_t precision, double scale,
+                   char *err_buf, size_t err_buf_len);
+
+/* ── Gate plans (replace ffi_cuda.h:27-32, 49-62) ────────────────────────── */
+
+/* Build a plan for a k-qubit gate.  matrix is row-major K x K (K = 2^k,
+ * matrix_len == K*K); qubits must be strictly ascending (canonical order,
+ * src/types/gate.rs:483-521) — local basis bit b addresses global qubit
+ * qubits[b] (cuda_gen.cpp:241-246).  1 <= k <= 6. */
+int cast_hip_gate_plan_create(const cast_hip_kernel_gen_spec_t *spec,
+                              const cast_complex64_t *matrix, size_t matrix_len,
+                              const uint32_t *qubits, size_t n_qubits, void **out_plan,
+                              char *err_buf, size_t err_buf_len);
+void cast_hip_gate_plan_destroy(void *plan);
+int cast_hip_gate_plan_info(const void *plan, cast_hip_gate_plan_info_t *out, char *err_buf,
+                            size_t err_buf_len);
+
+/* Enqueue the gate on `stream` (NULL = null stream) against the statevector
+ * at sv_dptr holding 2^sv_n_qubits amplitudes of the plan's precision.
+ * Replaces cast_cuda_kernel_launch (ffi_cuda.h:59-61, cuda_exec.cpp:302-335).
+ * Non-blocking. */
+int cast_hip_gate_apply(const void *plan, void *stream, uint64_t sv_dptr, uint32_t sv_n_qubits,
+                        char *err_buf, size_t err_buf_len);
+
+/* Enqueue n_items independent (plan, statevector) pairs as ONE launch: item i
+ * applies plans[i] to svs[i] (2^sv_n_qubits[i] amplitudes).  All plans must
+ * share precision and k.  Used by batched trajectories and by serving.
+ * No reference counterpart (the reference launches one kernel per apply). */
+int cast_hip_gate_apply_batch(const void *const *plans, const uint64_t *svs,
+                              const uint32_t *sv_n_qubits, size_t n_items, void *stream,
+                              char *err_buf, size_t err_buf_len);
+
+/* ⟨Z_a Z_b⟩ for n_pairs qubit pairs (pairs = [a0,b0,a1,b1,...]), accumulated
+ * in f64 over |amp|^2; no normalisation is applied.  Synchronous on `stream`.
+ * No reference counterpart (the reference downloads the whole statevector,
+ * src/simulator/mod.rs:282-291). */
+int cast_hip_expect_zz(uint64_t sv_dptr, uint32_t sv_n_qubits, uint8_t precision,
+                       const uint32_t *pairs, size_t n_pairs, double *out, void *stream,
+                       char *err_buf, size_t err_buf_len);
+
+/* Library version string (static storage). */
+const char *cast_hip_version(void);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* CAST_HIP_H */
