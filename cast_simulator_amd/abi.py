@@ -18,7 +18,7 @@ LIB_PATH = Path(__file__).resolve().parent / "lib" / "libcast_hip.so"
 ERR_LEN = 1024
 
 F32, F64 = 0, 1
-VARIANT_AUTO, VARIANT_DENSE, VARIANT_MONOMIAL, VARIANT_LDS, VARIANT_MFMA = 0, 1, 2, 3, 4
+VARIANT_AUTO, VARIANT_DENSE, VARIANT_MONOMIAL, VARIANT_TILE, VARIANT_MFMA = 0, 1, 2, 3, 4
 
 
 class AbiError(RuntimeError):
@@ -95,6 +95,7 @@ _SIGS = {
     ),
     "cast_hip_gate_plan_destroy": (None, [_vp]),
     "cast_hip_gate_plan_info": (_i, [_vp, C.POINTER(PlanInfo), _err, _sz]),
+    "cast_hip_gate_plan_family": (_i, [_vp, _u32, C.POINTER(_u32), _err, _sz]),
     "cast_hip_gate_apply": (_i, [_vp, _vp, _u64, _u32, _err, _sz]),
     "cast_hip_gate_apply_batch": (
         _i, [C.POINTER(_vp), C.POINTER(_u64), C.POINTER(_u32), _sz, _vp, _err, _sz]),
@@ -208,6 +209,12 @@ def plan_info(plan: int) -> PlanInfo:
     info = PlanInfo()
     _call("cast_hip_gate_plan_info", plan, C.byref(info))
     return info
+
+
+def plan_family(plan: int, n_qubits: int) -> int:
+    out = _u32()
+    _call("cast_hip_gate_plan_family", plan, n_qubits, C.byref(out))
+    return out.value
 
 
 def gate_apply(plan: int, dptr: int, n_qubits: int, stream: int | None = None) -> None:

@@ -158,9 +158,12 @@ GatePlan *build_plan(const cast_hip_kernel_gen_spec_t *spec, const cast_complex6
     variant = monomial ? CAST_HIP_VARIANT_MONOMIAL : CAST_HIP_VARIANT_DENSE;
   if (variant == CAST_HIP_VARIANT_MONOMIAL && !monomial)
     throw Error("variant MONOMIAL requested for a matrix with >1 nonzero in a row");
-  if (variant == CAST_HIP_VARIANT_LDS || variant == CAST_HIP_VARIANT_MFMA)
-    variant = CAST_HIP_VARIANT_DENSE; // not yet specialised: dense handles every matrix
+  if (variant == CAST_HIP_VARIANT_TILE && n_qubits > 5)
+    variant = CAST_HIP_VARIANT_DENSE; // wave tiles exist for k <= 5
+  if (variant == CAST_HIP_VARIANT_MFMA && (n_qubits < 4 || n_qubits > 5))
+    variant = CAST_HIP_VARIANT_DENSE; // matrix-core tiles exist for k = 4, 5
   plan->variant = variant;
+  plan->forced = spec->variant != CAST_HIP_VARIANT_AUTO;
 
   plan->h_mat = new double[2 * K * K];
   std::copy(m.begin(), m.end(), plan->h_mat);
@@ -177,6 +180,24 @@ GatePlan *build_plan(const cast_hip_kernel_gen_spec_t *spec, const cast_complex6
   } else {
     check(hipMemcpy(plan->d_mat, m.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy(matrix)");
   }
+  if (variant != CAST_HIP_VARIANT_MONOMIAL && (n_qubits == 4 || n_qubits == 5)) {
+    // Real matrices of the three-product form: Re M, Im M, Re M + Im M.
+    const size_t KK = K * K;
+    std::vector<double> m3(3 * KK);
+    for (size_t i = 0; i < KK; ++i) {
+      m3[i] = m[2 * i];
+      m3[KK + i] = m[2 * i + 1];
+      m3[2 * KK + i] = m[2 * i] + m[2 * i + 1];
+    }
+    const size_t b3 = 3 * KK * (fp32 ? 4 : 8);
+    check(hipMalloc(&plan->d_mat3, b3), "hipMalloc(gate matrix, 3 real)");
+    if (fp32) {
+      std::vector<float> f(m3.begin(), m3.end());
+      check(hipMemcpy(plan->d_mat3, f.data(), b3, hipMemcpyHostToDevice), "hipMemcpy(m3)");
+    } else {
+      check(hipMemcpy(plan->d_mat3, m3.data(), b3, hipMemcpyHostToDevice), "hipMemcpy(m3)");
+    }
+  }
   return plan.release();
 }
 
@@ -185,6 +206,8 @@ void free_plan(GatePlan *p) {
     return;
   if (p->d_mat)
     (void)hipFree(p->d_mat);
+  if (p->d_mat3)
+    (void)hipFree(p->d_mat3);
   delete[] p->h_mat;
   delete p;
 }
@@ -421,6 +444,18 @@ int cast_hip_gate_plan_info(const void *plan, cast_hip_gate_plan_info_t *out, ch
     out->scalar_nnz = p->scalar_nnz;
     for (uint32_t b = 0; b < p->k; ++b)
       out->qubits[b] = p->q[b];
+  });
+}
+
+int cast_hip_gate_plan_family(const void *plan, uint32_t sv_n_qubits, uint32_t *out_variant,
+                              char *err, size_t len) {
+  return guard(err, len, [&] {
+    if (!plan || !out_variant)
+      throw Error("null argument");
+    const GatePlan *p = static_cast<const GatePlan *>(plan);
+    if (sv_n_qubits < p->k || sv_n_qubits > 40)
+      throw Error("statevector qubit count out of range for this gate");
+    *out_variant = plan_family(*p, sv_n_qubits);
   });
 }
 

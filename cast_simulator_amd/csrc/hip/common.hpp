@@ -72,10 +72,12 @@ struct GateGeom {
 struct GatePlan {
   uint32_t k = 0;
   uint8_t precision = 1;
-  uint32_t variant = 0;
+  uint32_t variant = 0;  // CAST_HIP_VARIANT_* (DENSE = dense family, launcher picks)
+  bool forced = false;   // variant requested explicitly by the spec
   uint32_t q[8] = {};
   int device = 0;
   void *d_mat = nullptr;       // K*K complex, interleaved, device precision
+  void *d_mat3 = nullptr;      // [Re M | Im M | Re M + Im M], 3*K*K real (k = 4, 5)
   double *h_mat = nullptr;     // snapped matrix in f64 (host copy, 2*K*K)
   uint64_t scalar_nnz = 0;
   // Monomial form: rows that differ from identity, each with one source col.
@@ -88,6 +90,8 @@ struct GatePlan {
 
 // Launch entry points implemented in gate_kernels.hip.
 void launch_gate(const GatePlan &plan, hipStream_t stream, void *sv, uint32_t n_qubits);
+// Kernel family launch_gate uses for `plan` on an n-qubit statevector.
+uint32_t plan_family(const GatePlan &plan, uint32_t n_qubits);
 struct BatchItem {
   void *sv;
   const void *mat;

@@ -108,6 +108,132 @@ __global__ void __launch_bounds__(256) k_dense(T *__restrict__ sv, const T *__re
     dense_update<T, KL>(amp, mat, combo_base<KL>(c, g.q), delta);
 }
 
+// ── Wave tiles with cross-lane exchange of low-order targets ───────────────
+//
+// A wave owns a tile of 64 * K amplitudes: the 6 lowest address bits (one per
+// lane), plus K register slots whose bits are, during load and store, the
+// first L free (non-target) positions >= 6 ("F") followed by the targets
+// >= 6.  Every load and store instruction therefore moves 64 consecutive
+// amplitudes (1 KiB at f64) and the streaming hint is safe.  The L targets
+// below 6 live in lane bits after the load; each is exchanged with register
+// bit j (xor-shuffle across lanes, one exchange per register pair) so that
+// register bits then enumerate exactly the K target states of one combo per
+// lane.  Outputs are produced 2^L rows at a time, exchanged back, and stored.
+
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef double d2v __attribute__((ext_vector_type(2)));
+template <typename T> struct NV;
+template <> struct NV<float> { using type = f2v; };
+template <> struct NV<double> { using type = d2v; };
+
+template <typename T>
+__device__ __forceinline__ typename NV<T>::type ld_stream(const T *p) {
+  return __builtin_nontemporal_load(reinterpret_cast<const typename NV<T>::type *>(p));
+}
+template <typename T>
+__device__ __forceinline__ void st_stream(T *p, typename NV<T>::type v) {
+  __builtin_nontemporal_store(v, reinterpret_cast<typename NV<T>::type *>(p));
+}
+
+struct TileGeom {
+  uint64_t n_tiles;    // 2^(n - 6 - k)
+  uint32_t ins[6];     // ascending insert positions (minus 6) of F and targets >= 6
+  uint32_t reg_pos[6]; // address bit of register bit b during load/store
+  uint32_t low_t[6];   // targets below 6, ascending
+};
+
+template <typename T, int KL, int L>
+__global__ void __launch_bounds__(256) k_tile(T *__restrict__ sv, const T *__restrict__ mat,
+                                              TileGeom g) {
+  constexpr int K = 1 << KL, G = 1 << L;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t tile =
+      ((uint64_t(blockIdx.y) * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  if (tile >= g.n_tiles)
+    return; // whole wave
+  uint64_t tb = tile;
+#pragma unroll
+  for (int b = 0; b < KL; ++b)
+    tb = insert_zero(tb, g.ins[b]);
+  T *__restrict__ p = sv + 2 * ((tb << 6) | lane);
+
+  T vr[K], vi[K];
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    const auto x = ld_stream(p + 2 * local_delta<KL>(uint32_t(i), g.reg_pos));
+    vr[i] = x.x;
+    vi[i] = x.y;
+  }
+  // lane bit low_t[j]  <->  register bit j
+#pragma unroll
+  for (int j = 0; j < L; ++j) {
+    const int m = 1 << g.low_t[j];
+    const bool up = (lane >> g.low_t[j]) & 1u;
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+      if (r & (1 << j))
+        continue;
+      const int r1 = r | (1 << j);
+      const T sr = up ? vr[r] : vr[r1], si = up ? vi[r] : vi[r1];
+      const T gr = __shfl_xor(sr, m, 64), gi = __shfl_xor(si, m, 64);
+      if (up) {
+        vr[r] = gr;
+        vi[r] = gi;
+      } else {
+        vr[r1] = gr;
+        vi[r1] = gi;
+      }
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < K / G; ++h) {
+    T orr[G], oi[G];
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+      const int r = h * G + q;
+      T ar0 = T(0), ar1 = T(0), ai = T(0);
+#pragma unroll
+      for (int c = 0; c < K; ++c) {
+        const T mr = mat[2 * (r * K + c)];
+        const T mi = mat[2 * (r * K + c) + 1];
+        ar0 = __builtin_fma(mr, vr[c], ar0);
+        ar1 = __builtin_fma(mi, vi[c], ar1);
+        ai = __builtin_fma(mr, vi[c], ai);
+        ai = __builtin_fma(mi, vr[c], ai);
+      }
+      orr[q] = ar0 - ar1;
+      oi[q] = ai;
+    }
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+      const int m = 1 << g.low_t[j];
+      const bool up = (lane >> g.low_t[j]) & 1u;
+#pragma unroll
+      for (int q = 0; q < G; ++q) {
+        if (q & (1 << j))
+          continue;
+        const int q1 = q | (1 << j);
+        const T sr = up ? orr[q] : orr[q1], si = up ? oi[q] : oi[q1];
+        const T gr = __shfl_xor(sr, m, 64), gi = __shfl_xor(si, m, 64);
+        if (up) {
+          orr[q] = gr;
+          oi[q] = gi;
+        } else {
+          orr[q1] = gr;
+          oi[q1] = gi;
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+      typename NV<T>::type o;
+      o.x = orr[q];
+      o.y = oi[q];
+      st_stream(p + 2 * local_delta<KL>(uint32_t(h * G + q), g.reg_pos), o);
+    }
+  }
+}
+
 // ── Monomial: at most one nonzero per row ───────────────────────────────────
 
 template <typename T, int KL> struct MonoArgs {
@@ -140,6 +266,98 @@ __global__ void __launch_bounds__(256) k_monomial(T *__restrict__ sv, MonoArgs<T
         o.y = m.re[j] * x[j].y + m.im[j] * x[j].x;
         amp[base + local_delta<KL>(m.row[j], g.q)] = o;
       }
+  }
+}
+
+// ── Matrix cores: complex matvec as three real MFMA products ───────────────
+//
+// For a tile of 16 consecutive combos the update is the complex product
+// Out(K x 16) = M(K x K) * V(K x 16).  It is computed with three real
+// products (Gauss):  t1 = Mr Vr,  t2 = Mi Vi,  t3 = (Mr + Mi)(Vr + Vi),
+// Re Out = t1 - t2,  Im Out = t3 - t1 - t2  — 3 K^2 instead of 4 K^2
+// multiply-adds.  Each product runs on v_mfma_*_16x16x4: lane l supplies
+// A[l % 16][l / 16] and B[l / 16][l % 16] and receives, in accumulator
+// register i, D[4 (l / 16) + i][l % 16] (f32) or D[l / 16 + 4 i][l % 16]
+// (f64) — layouts confirmed on gfx950 with a one-hot probe.  So lane l owns
+// combo (l % 16) of the tile, loads the input amplitudes c = 4 s + l / 16
+// (s = 0 .. K/4-1) and stores the rows its accumulators hold.  The three real
+// matrices are loaded into registers once per wave and reused for every tile
+// the wave processes.
+
+template <typename T> struct Acc4;
+template <> struct Acc4<float> {
+  using type = float __attribute__((ext_vector_type(4)));
+};
+template <> struct Acc4<double> {
+  using type = double __attribute__((ext_vector_type(4)));
+};
+
+__device__ __forceinline__ Acc4<float>::type mfma16(float a, float b, Acc4<float>::type c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ Acc4<double>::type mfma16(double a, double b, Acc4<double>::type c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+template <typename T, int KL>
+__global__ void __launch_bounds__(256) k_mfma(T *__restrict__ sv, const T *__restrict__ m3,
+                                              GateGeom g, uint64_t n_tiles) {
+  constexpr int K = 1 << KL, RB = K / 16, KS = K / 4;
+  static_assert(RB >= 1, "MFMA path needs k >= 4");
+  using VT = typename V2<T>::type;
+  using AV = typename Acc4<T>::type;
+  VT *__restrict__ amp = reinterpret_cast<VT *>(sv);
+  const unsigned lane = threadIdx.x & 63u, j = lane >> 4, col = lane & 15u;
+
+  T a[3][RB][KS];
+#pragma unroll
+  for (int m = 0; m < 3; ++m)
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+      for (int s = 0; s < KS; ++s)
+        a[m][rb][s] = m3[m * K * K + (16 * rb + col) * K + 4 * s + j];
+  uint64_t din[KS], dout[RB][4];
+#pragma unroll
+  for (int s = 0; s < KS; ++s)
+    din[s] = local_delta<KL>(4 * s + j, g.q);
+  // Accumulator register i of lane l holds row 4 (l / 16) + i for f32 and
+  // row (l / 16) + 4 i for f64 (v_mfma_f64_16x16x4 interleaves its rows).
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      dout[rb][i] = local_delta<KL>(16 * rb + (sizeof(T) == 8 ? j + 4 * i : 4 * j + i), g.q);
+
+  const uint64_t n_waves = (uint64_t(gridDim.x) * blockDim.x) >> 6;
+  for (uint64_t t = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6; t < n_tiles;
+       t += n_waves) {
+    const uint64_t base = combo_base<KL>(t * 16 + col, g.q);
+    T br[KS], bi[KS], bs[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const VT x = amp[base + din[s]];
+      br[s] = x.x;
+      bi[s] = x.y;
+      bs[s] = x.x + x.y;
+    }
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+      AV t1 = {0, 0, 0, 0}, t2 = {0, 0, 0, 0}, t3 = {0, 0, 0, 0};
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        t1 = mfma16(a[0][rb][s], br[s], t1);
+        t2 = mfma16(a[1][rb][s], bi[s], t2);
+        t3 = mfma16(a[2][rb][s], bs[s], t3);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        VT o;
+        o.x = t1[i] - t2[i];
+        o.y = t3[i] - t1[i] - t2[i];
+        amp[base + dout[rb][i]] = o;
+      }
+    }
   }
 }
 
@@ -220,9 +438,160 @@ void launch_mono_t(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &g
   check(hipGetLastError(), "hipLaunchKernelGGL(k_monomial)");
 }
 
+template <typename T, int KL>
+void launch_mfma_t(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &g) {
+  static const unsigned per_cu = [] {
+    const char *e = std::getenv("CAST_HIP_MFMA_BLOCKS_PER_CU");
+    long v = e && *e ? std::strtol(e, nullptr, 10) : 8;
+    return unsigned(v > 0 && v <= 1024 ? v : 8);
+  }();
+  const uint64_t tiles = g.combos / 16;
+  const uint64_t need = (tiles + 3) / 4; // 4 waves per block
+  const uint64_t cap = uint64_t(std::max(1, device_props().n_cu)) * per_cu;
+  const unsigned grid = unsigned(std::max<uint64_t>(1, std::min(need, cap)));
+  hipLaunchKernelGGL((k_mfma<T, KL>), dim3(grid), dim3(kBlock), 0, s, static_cast<T *>(sv),
+                     static_cast<const T *>(p.d_mat3), g, tiles);
+  check(hipGetLastError(), "hipLaunchKernelGGL(k_mfma)");
+}
+
+template <typename T, int KL, int L>
+void launch_tile_kl(const GatePlan &p, hipStream_t s, void *sv, const TileGeom &tg) {
+  const uint64_t blocks = (tg.n_tiles + 3) / 4; // 4 waves per block, one tile per wave
+  hipLaunchKernelGGL((k_tile<T, KL, L>), full_grid(blocks * kBlock), dim3(kBlock), 0, s,
+                     static_cast<T *>(sv), static_cast<const T *>(p.d_mat), tg);
+  check(hipGetLastError(), "hipLaunchKernelGGL(k_tile)");
+}
+
+template <typename T, int KL>
+void launch_tile_k(const GatePlan &p, hipStream_t s, void *sv, const TileGeom &tg, uint32_t L) {
+  switch (L) {
+  case 0:
+    return launch_tile_kl<T, KL, 0>(p, s, sv, tg);
+  case 1:
+    if constexpr (KL >= 1)
+      return launch_tile_kl<T, KL, 1>(p, s, sv, tg);
+    break;
+  case 2:
+    if constexpr (KL >= 2)
+      return launch_tile_kl<T, KL, 2>(p, s, sv, tg);
+    break;
+  case 3:
+    if constexpr (KL >= 3)
+      return launch_tile_kl<T, KL, 3>(p, s, sv, tg);
+    break;
+  case 4:
+    if constexpr (KL >= 4)
+      return launch_tile_kl<T, KL, 4>(p, s, sv, tg);
+    break;
+  case 5:
+    if constexpr (KL >= 5)
+      return launch_tile_kl<T, KL, 5>(p, s, sv, tg);
+    break;
+  default:
+    break;
+  }
+  throw Error("wave-tile kernel: invalid low-target count");
+}
+
+// Wave-tile geometry for a gate on an n-qubit statevector (n >= k + 6).
+TileGeom tile_geom(const GatePlan &p, uint32_t n, uint32_t *n_low) {
+  TileGeom tg{};
+  tg.n_tiles = uint64_t(1) << (n - 6 - p.k);
+  uint32_t L = 0, h = 0, high[6] = {};
+  for (uint32_t b = 0; b < p.k; ++b) {
+    if (p.q[b] < 6)
+      tg.low_t[L++] = p.q[b];
+    else
+      high[h++] = p.q[b];
+  }
+  // F: the first L non-target positions >= 6.
+  uint32_t f[6] = {}, nf = 0;
+  for (uint32_t pos = 6; nf < L; ++pos) {
+    bool is_t = false;
+    for (uint32_t b = 0; b < p.k; ++b)
+      is_t = is_t || p.q[b] == pos;
+    if (!is_t)
+      f[nf++] = pos;
+  }
+  for (uint32_t j = 0; j < L; ++j)
+    tg.reg_pos[j] = f[j];
+  for (uint32_t j = 0; j < h; ++j)
+    tg.reg_pos[L + j] = high[j];
+  uint32_t ins[6];
+  for (uint32_t b = 0; b < p.k; ++b)
+    ins[b] = tg.reg_pos[b];
+  std::sort(ins, ins + p.k);
+  for (uint32_t b = 0; b < p.k; ++b)
+    tg.ins[b] = ins[b] - 6;
+  *n_low = L;
+  return tg;
+}
+
+template <typename T>
+void launch_tile(const GatePlan &p, hipStream_t s, void *sv, uint32_t n) {
+  uint32_t L = 0;
+  const TileGeom tg = tile_geom(p, n, &L);
+  switch (p.k) {
+  case 1:
+    return launch_tile_k<T, 1>(p, s, sv, tg, L);
+  case 2:
+    return launch_tile_k<T, 2>(p, s, sv, tg, L);
+  case 3:
+    return launch_tile_k<T, 3>(p, s, sv, tg, L);
+  case 4:
+    return launch_tile_k<T, 4>(p, s, sv, tg, L);
+  case 5:
+    return launch_tile_k<T, 5>(p, s, sv, tg, L);
+  default:
+    throw Error("wave-tile kernel supports k <= 5");
+  }
+}
+
+// Kernel family for a dense plan on an n-qubit statevector.  A forced family
+// falls back to the direct kernel where its shape requirement is not met
+// (wave tiles need n >= k + 6, matrix-core tiles k in {4, 5} and n >= k + 4).
+uint32_t dense_family(const GatePlan &p, uint32_t n) {
+  const bool tile_ok = p.k <= 5 && n >= p.k + 6;
+  const bool mfma_ok = p.d_mat3 != nullptr && n >= p.k + 4;
+  uint32_t n_low = 0;
+  for (uint32_t b = 0; b < p.k; ++b)
+    n_low += p.q[b] < 6;
+  uint32_t v = p.variant;
+  if (v == 0 || (v == 1 && !p.forced)) {
+    // Automatic choice, from 30-qubit measurements on MI355X:
+    //  * f32, k >= 4: the VALU kernels are FMA-bound without packed FP32;
+    //    the matrix-core three-product kernel is not.
+    //  * wave tiles stream full 1 KiB rows and win for up to 3 low-order
+    //    targets; with 4 or 5 the exchanged register groups cost more than
+    //    the matrix-core kernel's 256-byte segments.
+    if (mfma_ok && (p.precision == 0 || n_low >= 4))
+      v = 4;
+    else if (tile_ok && n_low <= 3)
+      v = 3;
+    else
+      v = 1;
+  }
+  if (v == 3 && !tile_ok)
+    v = 1;
+  if (v == 4 && !mfma_ok)
+    v = tile_ok ? 3 : 1;
+  return v;
+}
+
 template <typename T>
 void launch_by_k(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &g) {
   const bool mono = p.variant == 2;
+  if (!mono) {
+    const uint32_t n = p.k + uint32_t(63 - __builtin_clzll(g.combos));
+    const uint32_t fam = dense_family(p, n);
+    if (fam == 4) {
+      if (p.k == 4)
+        return launch_mfma_t<T, 4>(p, s, sv, g);
+      return launch_mfma_t<T, 5>(p, s, sv, g);
+    }
+    if (fam == 3)
+      return launch_tile<T>(p, s, sv, n);
+  }
 #define CAST_HIP_CASE(KL)                                                                        \
   case KL:                                                                                       \
     if (mono)                                                                                    \
@@ -269,6 +638,12 @@ void launch_batch_by_k(const BatchArgs &a, unsigned n, uint32_t k, uint64_t max_
 }
 
 } // namespace
+
+uint32_t plan_family(const GatePlan &plan, uint32_t n_qubits) {
+  if (plan.variant == 2)
+    return 2;
+  return dense_family(plan, n_qubits);
+}
 
 void launch_gate(const GatePlan &plan, hipStream_t stream, void *sv, uint32_t n_qubits) {
   if (n_qubits < plan.k)

@@ -362,12 +362,26 @@ PYBIND11_MODULE(_host, m) {
                              py::arg("max_s") = t.max_s);
            })
       .def("plan_info", &HipKernelManager::plan_info)
-      .def("n_kernels", &HipKernelManager::n_kernels);
+      .def("family", &HipKernelManager::family)
+      .def("n_kernels", &HipKernelManager::n_kernels)
+      .def("set_autotune", &HipKernelManager::set_autotune, py::arg("on"),
+           py::arg("min_qubits") = 22, py::arg("samples") = 2)
+      .def("autotune", &HipKernelManager::autotune)
+      .def("tune", &HipKernelManager::tune, py::arg("ids"), py::arg("sv"),
+           py::arg("max_rounds") = 8)
+      .def("tuning_report", [](const HipKernelManager &m) {
+        py::list out;
+        for (const auto &e : m.tuning_report())
+          out.append(py::dict(py::arg("id") = e.id, py::arg("n_qubits") = e.n_qubits,
+                              py::arg("chosen") = e.chosen_family,
+                              py::arg("best_s") = e.best_s));
+        return out;
+      });
 
   py::class_<Simulator>(m, "Simulator")
       .def(py::init<HipKernelGenSpec>())
-      .def_static("f32", &Simulator::f32)
-      .def_static("f64", &Simulator::f64)
+      .def_static("f32", [] { return std::make_unique<Simulator>(HipKernelGenSpec::f32()); })
+      .def_static("f64", [] { return std::make_unique<Simulator>(HipKernelGenSpec::f64()); })
       .def("compile", &Simulator::compile)
       .def("run",
            [](Simulator &s, const CircuitGraph &g, HipStatevector &sv) {

@@ -119,6 +119,53 @@ HipKernelManager::~HipKernelManager() {
     cast_hip_stream_destroy(stream_);
   for (auto &kv : plans_)
     cast_hip_gate_plan_destroy(kv.second);
+  for (auto &kv : tune_)
+    for (void *p : kv.second.plans)
+      cast_hip_gate_plan_destroy(p);
+}
+
+void HipKernelManager::set_autotune(bool on, uint32_t min_qubits, uint32_t samples) {
+  std::lock_guard<std::mutex> lock(mu_);
+  autotune_ = on;
+  tune_min_qubits_ = min_qubits;
+  tune_samples_ = std::max<uint32_t>(1, samples);
+}
+
+HipKernelManager::TuneState &HipKernelManager::tune_state_locked(KernelId id, uint32_t n) {
+  auto key = std::make_pair(id, n);
+  auto it = tune_.find(key);
+  if (it != tune_.end())
+    return it->second;
+  TuneState st;
+  const QuantumGate &g = *gates_.at(id);
+  const auto &d = g.matrix().data();
+  std::vector<cast_complex64_t> m(d.size());
+  for (size_t i = 0; i < d.size(); ++i)
+    m[i] = {d[i].real(), d[i].imag()};
+  char err[kErr];
+  for (uint32_t v : {CAST_HIP_VARIANT_TILE, CAST_HIP_VARIANT_MFMA, CAST_HIP_VARIANT_DENSE}) {
+    cast_hip_kernel_gen_spec_t s{};
+    s.precision = spec_.precision;
+    s.ztol = spec_.ztol;
+    s.otol = spec_.otol;
+    s.variant = v;
+    void *plan = nullptr;
+    throw_if(cast_hip_gate_plan_create(&s, m.data(), m.size(), g.qubits().data(),
+                                       g.qubits().size(), &plan, err, kErr),
+             err);
+    uint32_t fam = 0;
+    if (cast_hip_gate_plan_family(plan, n, &fam, err, kErr) != 0 ||
+        std::find(st.fams.begin(), st.fams.end(), fam) != st.fams.end()) {
+      cast_hip_gate_plan_destroy(plan); // inapplicable here, or same kernel as another
+      continue;
+    }
+    st.fams.push_back(fam);
+    st.plans.push_back(plan);
+  }
+  st.samples.resize(st.fams.size());
+  if (st.fams.size() <= 1)
+    st.chosen = 0;
+  return tune_.emplace(key, std::move(st)).first->second;
 }
 
 KernelId HipKernelManager::generate(const QuantumGate &gate) {
@@ -153,6 +200,7 @@ KernelId HipKernelManager::generate(const QuantumGate &gate) {
   }
   const KernelId id = next_id_++;
   plans_[id] = plan;
+  gates_[id] = std::make_shared<QuantumGate>(gate);
   dedup_[key] = id;
   return id;
 }
@@ -171,7 +219,66 @@ void HipKernelManager::apply(KernelId id, HipStatevector &sv) {
     throw Error("statevector precision does not match kernel precision");
   if (info.qubits[info.n_qubits - 1] >= sv.n_qubits())
     throw Error("gate acts on a qubit outside the statevector");
-  pending_.push_back({id, sv.dptr(), sv.n_qubits()});
+  const void *plan = it->second;
+  int cand = -1;
+  // Tuning applies to dense plans of the automatic family on large states.
+  if (autotune_ && spec_.variant == CAST_HIP_VARIANT_AUTO && sv.n_qubits() >= tune_min_qubits_ &&
+      info.variant != CAST_HIP_VARIANT_MONOMIAL) {
+    TuneState &st = tune_state_locked(id, sv.n_qubits());
+    if (st.chosen >= 0) {
+      plan = st.plans[size_t(st.chosen)];
+    } else {
+      size_t best = 0;
+      for (size_t c = 1; c < st.samples.size(); ++c)
+        if (st.samples[c].size() < st.samples[best].size())
+          best = c;
+      cand = int(best);
+      plan = st.plans[best];
+    }
+  }
+  pending_.push_back({id, sv.dptr(), sv.n_qubits(), plan, cand});
+}
+
+uint32_t HipKernelManager::tune(const std::vector<KernelId> &ids, HipStatevector &sv,
+                                uint32_t max_rounds) {
+  uint32_t rounds = 0;
+  for (; rounds < max_rounds; ++rounds) {
+    bool pending_tune = false;
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      if (!autotune_ || sv.n_qubits() < tune_min_qubits_)
+        return rounds;
+      for (KernelId id : ids) {
+        auto it = tune_.find(std::make_pair(id, sv.n_qubits()));
+        if (it == tune_.end() || it->second.chosen < 0)
+          pending_tune = true;
+      }
+    }
+    if (!pending_tune)
+      break;
+    for (KernelId id : ids)
+      apply(id, sv);
+    sync(true);
+  }
+  return rounds;
+}
+
+std::vector<TuneEntry> HipKernelManager::tuning_report() const {
+  std::lock_guard<std::mutex> lock(mu_);
+  std::vector<TuneEntry> out;
+  for (const auto &kv : tune_) {
+    TuneEntry e;
+    e.id = kv.first.first;
+    e.n_qubits = kv.first.second;
+    e.chosen_family = kv.second.chosen >= 0 ? int(kv.second.fams[size_t(kv.second.chosen)]) : -1;
+    for (size_t c = 0; c < kv.second.fams.size(); ++c) {
+      const auto &s = kv.second.samples[c];
+      e.best_s.emplace_back(kv.second.fams[c],
+                            s.empty() ? -1.0 : *std::min_element(s.begin(), s.end()));
+    }
+    out.push_back(std::move(e));
+  }
+  return out;
 }
 
 void HipKernelManager::ensure_stream_locked() {
@@ -213,6 +320,8 @@ SyncStats HipKernelManager::sync(bool timed) {
     stream = stream_;
     work.swap(pending_);
     char err[kErr];
+    for (const auto &w : work)
+      timed = timed || w.cand >= 0; // tuning samples are always timed
     if (timed) {
       ensure_events_locked(work.size() + 1);
       ev.assign(events_.begin(), events_.begin() + long(work.size() + 1));
@@ -220,8 +329,8 @@ SyncStats HipKernelManager::sync(bool timed) {
         throw_if(cast_hip_event_record(ev[0], stream, err, kErr), err);
     }
     for (size_t i = 0; i < work.size(); ++i) {
-      throw_if(cast_hip_gate_apply(plans_.at(work[i].id), stream, work[i].dptr,
-                                   work[i].n_qubits, err, kErr),
+      throw_if(cast_hip_gate_apply(work[i].plan, stream, work[i].dptr, work[i].n_qubits, err,
+                                   kErr),
                err);
       if (timed)
         throw_if(cast_hip_event_record(ev[i + 1], stream, err, kErr), err);
@@ -235,6 +344,29 @@ SyncStats HipKernelManager::sync(bool timed) {
       float ms = 0.f;
       throw_if(cast_hip_event_elapsed_ms(ev[i], ev[i + 1], &ms, err, kErr), err);
       st.kernels.push_back({work[i].id, double(ms) * 1e-3});
+    }
+    std::lock_guard<std::mutex> lock(mu_);
+    for (size_t i = 0; i < work.size(); ++i) {
+      if (work[i].cand < 0)
+        continue;
+      auto it = tune_.find(std::make_pair(work[i].id, work[i].n_qubits));
+      if (it == tune_.end() || it->second.chosen >= 0)
+        continue;
+      TuneState &ts = it->second;
+      ts.samples[size_t(work[i].cand)].push_back(st.kernels[i].gpu_s);
+      bool done = true;
+      for (const auto &s : ts.samples)
+        done = done && s.size() >= tune_samples_;
+      if (done) {
+        double best = 1e300;
+        for (size_t c = 0; c < ts.samples.size(); ++c) {
+          const double m = *std::min_element(ts.samples[c].begin(), ts.samples[c].end());
+          if (m < best) {
+            best = m;
+            ts.chosen = int(c);
+          }
+        }
+      }
     }
   }
   st.wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();
@@ -267,6 +399,20 @@ PlanInfo HipKernelManager::plan_info(KernelId id) const {
   p.scalar_nnz = info.scalar_nnz;
   p.qubits.assign(info.qubits, info.qubits + info.n_qubits);
   return p;
+}
+
+uint32_t HipKernelManager::family(KernelId id, uint32_t n_qubits) const {
+  std::lock_guard<std::mutex> lock(mu_);
+  auto tit = tune_.find(std::make_pair(id, n_qubits));
+  if (tit != tune_.end() && tit->second.chosen >= 0)
+    return tit->second.fams[size_t(tit->second.chosen)];
+  auto it = plans_.find(id);
+  if (it == plans_.end())
+    throw Error("kernel " + std::to_string(id) + " not found");
+  uint32_t v = 0;
+  char err[kErr];
+  throw_if(cast_hip_gate_plan_family(it->second, n_qubits, &v, err, kErr), err);
+  return v;
 }
 
 size_t HipKernelManager::n_kernels() const {

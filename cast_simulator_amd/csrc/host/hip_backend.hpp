@@ -20,6 +20,8 @@
 #include "circuit.hpp"
 #include "profile.hpp"
 
+#include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -88,6 +90,17 @@ struct PlanInfo {
   std::vector<uint32_t> qubits;
 };
 
+// Per-gate kernel autotuning: for a dense gate on a statevector of at least
+// `min_qubits` qubits, every applicable kernel family (wave tile, matrix
+// core, direct) is timed on the real gate, `samples` times each, and the
+// fastest (by minimum) is used from then on.  Keyed by (kernel id, n).
+struct TuneEntry {
+  KernelId id = 0;
+  uint32_t n_qubits = 0;
+  int chosen_family = -1; // CAST_HIP_VARIANT_*, -1 while tuning
+  std::vector<std::pair<uint32_t, double>> best_s; // (family, min seconds)
+};
+
 class HipKernelManager {
 public:
   explicit HipKernelManager(HipKernelGenSpec spec);
@@ -101,27 +114,52 @@ public:
   SyncStats sync(bool timed = true);
   TimingStats time_adaptive(KernelId id, HipStatevector &sv, double budget_s);
   PlanInfo plan_info(KernelId id) const;
+  // Kernel family (CAST_HIP_VARIANT_*) launched for `id` on n_qubits; with
+  // autotuning, the tuned family once decided.
+  uint32_t family(KernelId id, uint32_t n_qubits) const;
   size_t n_kernels() const;
   void *stream(); // created on first use
   const void *plan_handle(KernelId id) const;
+
+  // Autotuning controls.
+  void set_autotune(bool on, uint32_t min_qubits = 22, uint32_t samples = 2);
+  bool autotune() const { return autotune_; }
+  // Apply `ids` to `sv` repeatedly until every (id, n) is tuned (at most
+  // max_rounds passes).  Modifies sv; returns the passes run.
+  uint32_t tune(const std::vector<KernelId> &ids, HipStatevector &sv, uint32_t max_rounds = 8);
+  std::vector<TuneEntry> tuning_report() const;
 
 private:
   struct Pending {
     KernelId id;
     uint64_t dptr;
     uint32_t n_qubits;
+    const void *plan;
+    int cand; // tuning candidate index, -1 when not a tuning sample
+  };
+  struct TuneState {
+    std::vector<uint32_t> fams;
+    std::vector<void *> plans;
+    std::vector<std::vector<double>> samples;
+    int chosen = -1;
   };
   void ensure_stream_locked();
   void ensure_events_locked(size_t n);
+  TuneState &tune_state_locked(KernelId id, uint32_t n);
 
   HipKernelGenSpec spec_;
   mutable std::mutex mu_;
   std::unordered_map<KernelId, void *> plans_;
+  std::unordered_map<KernelId, std::shared_ptr<const QuantumGate>> gates_;
   std::unordered_map<std::string, KernelId> dedup_;
   KernelId next_id_ = 0;
   std::vector<Pending> pending_;
   void *stream_ = nullptr;
   std::vector<void *> events_;
+  bool autotune_ = true;
+  uint32_t tune_min_qubits_ = 22;
+  uint32_t tune_samples_ = 2;
+  std::map<std::pair<KernelId, uint32_t>, TuneState> tune_;
 };
 
 // Simulator<Hip>: state-vector simulation of a circuit graph.

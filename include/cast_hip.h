@@ -69,11 +69,16 @@ typedef struct cast_hip_kernel_gen_spec_t {
   uint32_t reserved;
 } cast_hip_kernel_gen_spec_t;
 
+/* AUTO: monomial when every row has <= 1 nonzero, else the dense family, whose
+ * kernel (direct / wave tile / matrix core) is picked per launch from k, the
+ * statevector size and the precision.  A forced variant falls back to the
+ * direct kernel where its shape requirement is not met. */
 #define CAST_HIP_VARIANT_AUTO 0u
-#define CAST_HIP_VARIANT_DENSE 1u    /* register-resident dense matvec, one combo per lane      */
+#define CAST_HIP_VARIANT_DENSE 1u    /* dense, one combo per lane, K amplitudes in registers    */
 #define CAST_HIP_VARIANT_MONOMIAL 2u /* <=1 nonzero per row (diag / permutation with phases)    */
-#define CAST_HIP_VARIANT_LDS 3u      /* low-order targets: coalesced tile staged through LDS    */
-#define CAST_HIP_VARIANT_MFMA 4u     /* dense k>=4 contraction on matrix cores                  */
+#define CAST_HIP_VARIANT_TILE 3u     /* dense, coalesced 64*K wave tiles; low-order targets      */
+                                     /* exchanged across lanes (k <= 5, n >= k + 6)            */
+#define CAST_HIP_VARIANT_MFMA 4u     /* dense k = 4, 5 on matrix cores, three real products     */
 
 /* Read-only description of a built plan (no reference counterpart: the
  * reference exposes this through PTX text, src/cuda/kernel.rs:847-882). */
@@ -167,6 +172,12 @@ int cast_hip_gate_plan_create(const cast_hip_kernel_gen_spec_t *spec,
 void cast_hip_gate_plan_destroy(void *plan);
 int cast_hip_gate_plan_info(const void *plan, cast_hip_gate_plan_info_t *out, char *err_buf,
                             size_t err_buf_len);
+
+/* Kernel family (CAST_HIP_VARIANT_DENSE, _MONOMIAL, _TILE or _MFMA) that
+ * cast_hip_gate_apply launches for this plan on a statevector of sv_n_qubits
+ * qubits.  No reference counterpart (diagnostics and benchmarks). */
+int cast_hip_gate_plan_family(const void *plan, uint32_t sv_n_qubits, uint32_t *out_variant,
+                              char *err_buf, size_t err_buf_len);
 
 /* Enqueue the gate on `stream` (NULL = null stream) against the statevector
  * at sv_dptr holding 2^sv_n_qubits amplitudes of the plan's precision.

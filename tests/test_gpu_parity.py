@@ -173,6 +173,40 @@ def test_dense_every_shape(qubits, f32):
     compare(g, n, f"dense {qubits}", f32)
 
 
+def apply_abi(matrix, qubits, init, f32, variant, n):
+    p = abi.F32 if f32 else abi.F64
+    plan = abi.plan_create(np.asarray(matrix), list(qubits), p, variant=variant)
+    d = abi.sv_alloc(n, p)
+    try:
+        abi.sv_upload(d, init, p)
+        abi.gate_apply(plan, d, n)
+        abi.stream_sync(None)
+        return abi.sv_download(d, n, p)
+    finally:
+        abi.sv_free(d)
+        abi.plan_destroy(plan)
+
+
+VARIANTS = [abi.VARIANT_AUTO, abi.VARIANT_DENSE, abi.VARIANT_TILE, abi.VARIANT_MFMA]
+
+
+@pytest.mark.parametrize("variant", VARIANTS, ids=["auto", "direct", "tile", "mfma"])
+@pytest.mark.parametrize("f32", [False, True], ids=["f64", "f32"])
+def test_every_kernel_family_every_shape(f32, variant):
+    """Each dense kernel family, forced through the C ABI, on every target
+    shape (low/high/mixed targets, k = 1..6) of a 14-qubit statevector and of
+    the smallest statevector each family accepts."""
+    for qubits in SHAPES:
+        g = cs.QuantumGate.random_unitary(qubits, 300 + sum(qubits))
+        for n in (14, len(qubits) + 6, max(qubits) + 1):
+            if max(qubits) >= n:
+                continue
+            init = seeded_state(n)
+            want = apply_oracle([g], init, f32)
+            got = apply_abi(g.matrix(), qubits, init, f32, variant, n)
+            assert_amps_close(want, got, TOL32 if f32 else TOL64, f"{qubits} n={n} v={variant}")
+
+
 @pytest.mark.parametrize("f32", [False, True])
 def test_dense_nonunitary_sparse(f32):
     for seed, q in enumerate([[0, 3], [1, 2, 6], [0, 4, 5, 9]]):
@@ -382,6 +416,31 @@ def test_noisy_gate_rejected_by_simulator():
     c.add(cs.QuantumGate.depolarizing(0, 0.1))
     with pytest.raises(cs.CastError, match="noisy"):
         cs.Simulator.f64().simulate(cs.CircuitGraph.from_circuit(c))
+
+
+# ── per-gate kernel autotuning ────────────────────────────────────────────
+
+@pytest.mark.parametrize("f32", [False, True])
+def test_autotuned_kernels_match_oracle(f32):
+    n = 22
+    gates = [cs.QuantumGate.random_unitary(q, 40 + i) for i, q in
+             enumerate([[0, 1, 2, 3], [3, 9, 14, 20, 21], [5, 17], [1, 2, 8, 16, 19]])]
+    mgr = cs.HipKernelManager(spec(f32))
+    mgr.set_autotune(True, min_qubits=22, samples=1)
+    ids = [mgr.generate(g) for g in gates]
+    scratch = cs.HipStatevector(n, prec(f32))
+    scratch.zero()
+    assert mgr.tune(ids, scratch) >= 1
+    report = mgr.tuning_report()
+    assert len(report) == len(ids) and all(e["chosen"] in (1, 3, 4) for e in report)
+    init = np.asarray(cs.random_statevector(n, 3))
+    sv = cs.HipStatevector(n, prec(f32))
+    sv.upload(init)
+    for i in ids:
+        mgr.apply(i, sv)
+    mgr.sync()
+    want = apply_oracle(gates, init, f32)
+    assert rel_l2(want, sv.download()) <= (1e-5 if f32 else 1e-12)
 
 
 # ── full-size properties (no host transfer of the 16 GiB state) ───────────
