@@ -32,7 +32,7 @@ sys.path.insert(0, str(ROOT))
 METRIC = "HBM GB/s per fused gate at 30q f64; batched circuits/s + p99 at fixed fidelity"
 HBM_PEAK_GBS = 8000.0
 SEED = 20260101
-KERNEL = {1: "k_dense", 2: "k_monomial", 3: "k_tile", 4: "k_mfma"}
+KERNEL = {1: "k_dense", 2: "k_monomial", 3: "k_tile", 4: "k_mfma", 5: "k_tile(swizzled)"}
 
 
 def log(msg: str) -> None:
@@ -75,7 +75,7 @@ def run_gpu(cs, n: int, precision: str, steps: int, warmup: int, max_size: int):
     # Setup (untimed, like the reference's compile phase): per-gate kernel
     # autotuning on the resident statevector.
     t_tune = time.perf_counter()
-    rounds = mgr.tune(ids, sv)
+    rounds = mgr.tune(ids, sv, 12)
     t_tune = time.perf_counter() - t_tune
     fam = {i: mgr.family(i, n) for i in set(ids)}
     log(f"  autotune: {rounds} passes, {t_tune:.1f} s")

@@ -35,8 +35,11 @@ class KernelGenSpec(C.Structure):
         ("ztol", C.c_double),
         ("otol", C.c_double),
         ("variant", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("flags", C.c_uint32),
     ]
+
+
+FLAG_TILE_SWIZZLE = 1
 
 
 class PlanInfo(C.Structure):
@@ -189,13 +192,13 @@ def scale(dptr: int, n_qubits: int, precision: int, factor: float) -> None:
 
 
 def plan_create(matrix: np.ndarray, qubits, precision: int, ztol: float | None = None,
-                otol: float | None = None, variant: int = VARIANT_AUTO) -> int:
+                otol: float | None = None, variant: int = VARIANT_AUTO, flags: int = 0) -> int:
     m = np.ascontiguousarray(matrix, dtype=np.complex128).reshape(-1)
     arr = (Complex64 * m.size)(*[(z.real, z.imag) for z in m])
     q = (_u32 * len(qubits))(*qubits)
     tol = 1e-6 if precision == F32 else 1e-12
     spec = KernelGenSpec(precision, tol if ztol is None else ztol, tol if otol is None else otol,
-                         variant, 0)
+                         variant, flags)
     out = _vp()
     _call("cast_hip_gate_plan_create", C.byref(spec), arr, m.size, q, len(qubits), C.byref(out))
     return out.value

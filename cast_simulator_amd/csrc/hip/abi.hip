@@ -164,6 +164,9 @@ GatePlan *build_plan(const cast_hip_kernel_gen_spec_t *spec, const cast_complex6
     variant = CAST_HIP_VARIANT_DENSE; // matrix-core tiles exist for k = 4, 5
   plan->variant = variant;
   plan->forced = spec->variant != CAST_HIP_VARIANT_AUTO;
+  if (spec->flags & ~CAST_HIP_FLAG_TILE_SWIZZLE)
+    throw Error("unknown flags " + std::to_string(spec->flags));
+  plan->flags = spec->flags;
 
   plan->h_mat = new double[2 * K * K];
   std::copy(m.begin(), m.end(), plan->h_mat);

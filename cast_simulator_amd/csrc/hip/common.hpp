@@ -74,6 +74,7 @@ struct GatePlan {
   uint8_t precision = 1;
   uint32_t variant = 0;  // CAST_HIP_VARIANT_* (DENSE = dense family, launcher picks)
   bool forced = false;   // variant requested explicitly by the spec
+  uint32_t flags = 0;    // CAST_HIP_FLAG_*
   uint32_t q[8] = {};
   int device = 0;
   void *d_mat = nullptr;       // K*K complex, interleaved, device precision

@@ -21,6 +21,7 @@
 //   k_dense_batch  k_dense over many (matrix, statevector) items, item =
 //                blockIdx.y, for batched trajectories and serving.
 
+#include "cast_hip.h"
 #include "common.hpp"
 
 #include <algorithm>
@@ -140,6 +141,7 @@ struct TileGeom {
   uint32_t ins[6];     // ascending insert positions (minus 6) of F and targets >= 6
   uint32_t reg_pos[6]; // address bit of register bit b during load/store
   uint32_t low_t[6];   // targets below 6, ascending
+  uint32_t swz;        // 0, or the number of low tile-index bits visited bit-reversed
 };
 
 template <typename T, int KL, int L>
@@ -147,10 +149,14 @@ __global__ void __launch_bounds__(256) k_tile(T *__restrict__ sv, const T *__res
                                               TileGeom g) {
   constexpr int K = 1 << KL, G = 1 << L;
   const uint32_t lane = threadIdx.x & 63u;
-  const uint64_t tile =
+  uint64_t tile =
       ((uint64_t(blockIdx.y) * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
   if (tile >= g.n_tiles)
     return; // whole wave
+  if (g.swz) { // bijection on [0, n_tiles): n_tiles is a power of two >= 2^swz
+    const uint64_t lo = tile & ((uint64_t(1) << g.swz) - 1);
+    tile = (tile ^ lo) | (__brevll(lo) >> (64 - g.swz));
+  }
   uint64_t tb = tile;
 #pragma unroll
   for (int b = 0; b < KL; ++b)
@@ -523,6 +529,8 @@ TileGeom tile_geom(const GatePlan &p, uint32_t n, uint32_t *n_low) {
   std::sort(ins, ins + p.k);
   for (uint32_t b = 0; b < p.k; ++b)
     tg.ins[b] = ins[b] - 6;
+  if (p.flags & CAST_HIP_FLAG_TILE_SWIZZLE)
+    tg.swz = std::min<uint32_t>(10, n - 6 - p.k);
   *n_low = L;
   return tg;
 }

@@ -143,23 +143,32 @@ HipKernelManager::TuneState &HipKernelManager::tune_state_locked(KernelId id, ui
   for (size_t i = 0; i < d.size(); ++i)
     m[i] = {d[i].real(), d[i].imag()};
   char err[kErr];
-  for (uint32_t v : {CAST_HIP_VARIANT_TILE, CAST_HIP_VARIANT_MFMA, CAST_HIP_VARIANT_DENSE}) {
+  const std::pair<uint32_t, uint32_t> cands[] = {
+      {CAST_HIP_VARIANT_TILE, 0u},
+      {CAST_HIP_VARIANT_TILE, CAST_HIP_FLAG_TILE_SWIZZLE},
+      {CAST_HIP_VARIANT_MFMA, 0u},
+      {CAST_HIP_VARIANT_DENSE, 0u}};
+  for (const auto &c : cands) {
     cast_hip_kernel_gen_spec_t s{};
     s.precision = spec_.precision;
     s.ztol = spec_.ztol;
     s.otol = spec_.otol;
-    s.variant = v;
+    s.variant = c.first;
+    s.flags = c.second;
     void *plan = nullptr;
     throw_if(cast_hip_gate_plan_create(&s, m.data(), m.size(), g.qubits().data(),
                                        g.qubits().size(), &plan, err, kErr),
              err);
     uint32_t fam = 0;
-    if (cast_hip_gate_plan_family(plan, n, &fam, err, kErr) != 0 ||
-        std::find(st.fams.begin(), st.fams.end(), fam) != st.fams.end()) {
+    const bool ok = cast_hip_gate_plan_family(plan, n, &fam, err, kErr) == 0;
+    // Candidate code: the family, or kTileSwizzled for the swizzled wave tiles.
+    const uint32_t code = (fam == CAST_HIP_VARIANT_TILE && c.second) ? kTileSwizzled : fam;
+    if (!ok || (c.second && fam != CAST_HIP_VARIANT_TILE) ||
+        std::find(st.fams.begin(), st.fams.end(), code) != st.fams.end()) {
       cast_hip_gate_plan_destroy(plan); // inapplicable here, or same kernel as another
       continue;
     }
-    st.fams.push_back(fam);
+    st.fams.push_back(code);
     st.plans.push_back(plan);
   }
   st.samples.resize(st.fams.size());

@@ -94,11 +94,15 @@ struct PlanInfo {
 // `min_qubits` qubits, every applicable kernel family (wave tile, matrix
 // core, direct) is timed on the real gate, `samples` times each, and the
 // fastest (by minimum) is used from then on.  Keyed by (kernel id, n).
+// Candidate code of the wave-tile kernel visiting tiles in swizzled order
+// (the other codes are the CAST_HIP_VARIANT_* families).
+constexpr uint32_t kTileSwizzled = 5;
+
 struct TuneEntry {
   KernelId id = 0;
   uint32_t n_qubits = 0;
-  int chosen_family = -1; // CAST_HIP_VARIANT_*, -1 while tuning
-  std::vector<std::pair<uint32_t, double>> best_s; // (family, min seconds)
+  int chosen_family = -1; // CAST_HIP_VARIANT_* or kTileSwizzled, -1 while tuning
+  std::vector<std::pair<uint32_t, double>> best_s; // (candidate code, min seconds)
 };
 
 class HipKernelManager {
@@ -115,7 +119,8 @@ public:
   TimingStats time_adaptive(KernelId id, HipStatevector &sv, double budget_s);
   PlanInfo plan_info(KernelId id) const;
   // Kernel family (CAST_HIP_VARIANT_*) launched for `id` on n_qubits; with
-  // autotuning, the tuned family once decided.
+  // autotuning, the tuned candidate once decided (kTileSwizzled for the
+  // swizzled wave tiles).
   uint32_t family(KernelId id, uint32_t n_qubits) const;
   size_t n_kernels() const;
   void *stream(); // created on first use

@@ -66,8 +66,13 @@ typedef struct cast_hip_kernel_gen_spec_t {
   double ztol;
   double otol;
   uint32_t variant;
-  uint32_t reserved;
+  uint32_t flags; /* CAST_HIP_FLAG_* */
 } cast_hip_kernel_gen_spec_t;
+
+/* Visit wave tiles in bit-reversed order of their low index bits.  Which
+ * order streams faster depends on how the gate's target bits fall on the
+ * HBM channel/bank mapping; the host autotuner tries both. */
+#define CAST_HIP_FLAG_TILE_SWIZZLE 1u
 
 /* AUTO: monomial when every row has <= 1 nonzero, else the dense family, whose
  * kernel (direct / wave tile / matrix core) is picked per launch from k, the

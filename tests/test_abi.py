@@ -63,6 +63,11 @@ def test_plan_rejects_oversized_gate():
     assert "exceeds the maximum of 6" in msg
 
 
+def test_plan_rejects_unknown_flags():
+    msg = _plan_err(matrix=np.eye(2), qubits=[0], precision=abi.F64, flags=0x80)
+    assert "unknown flags" in msg
+
+
 def test_plan_rejects_forced_monomial_on_dense_matrix():
     # classification runs on the host, before the device upload
     h = np.array([[1, 1], [1, -1]]) / np.sqrt(2)

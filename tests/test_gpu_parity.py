@@ -173,9 +173,9 @@ def test_dense_every_shape(qubits, f32):
     compare(g, n, f"dense {qubits}", f32)
 
 
-def apply_abi(matrix, qubits, init, f32, variant, n):
+def apply_abi(matrix, qubits, init, f32, variant, n, flags=0):
     p = abi.F32 if f32 else abi.F64
-    plan = abi.plan_create(np.asarray(matrix), list(qubits), p, variant=variant)
+    plan = abi.plan_create(np.asarray(matrix), list(qubits), p, variant=variant, flags=flags)
     d = abi.sv_alloc(n, p)
     try:
         abi.sv_upload(d, init, p)
@@ -187,12 +187,14 @@ def apply_abi(matrix, qubits, init, f32, variant, n):
         abi.plan_destroy(plan)
 
 
-VARIANTS = [abi.VARIANT_AUTO, abi.VARIANT_DENSE, abi.VARIANT_TILE, abi.VARIANT_MFMA]
+VARIANTS = [(abi.VARIANT_AUTO, 0), (abi.VARIANT_DENSE, 0), (abi.VARIANT_TILE, 0),
+            (abi.VARIANT_TILE, abi.FLAG_TILE_SWIZZLE), (abi.VARIANT_MFMA, 0)]
 
 
-@pytest.mark.parametrize("variant", VARIANTS, ids=["auto", "direct", "tile", "mfma"])
+@pytest.mark.parametrize("variant,flags", VARIANTS,
+                         ids=["auto", "direct", "tile", "tile-swizzled", "mfma"])
 @pytest.mark.parametrize("f32", [False, True], ids=["f64", "f32"])
-def test_every_kernel_family_every_shape(f32, variant):
+def test_every_kernel_family_every_shape(f32, variant, flags):
     """Each dense kernel family, forced through the C ABI, on every target
     shape (low/high/mixed targets, k = 1..6) of a 14-qubit statevector and of
     the smallest statevector each family accepts."""
@@ -203,7 +205,7 @@ def test_every_kernel_family_every_shape(f32, variant):
                 continue
             init = seeded_state(n)
             want = apply_oracle([g], init, f32)
-            got = apply_abi(g.matrix(), qubits, init, f32, variant, n)
+            got = apply_abi(g.matrix(), qubits, init, f32, variant, n, flags)
             assert_amps_close(want, got, TOL32 if f32 else TOL64, f"{qubits} n={n} v={variant}")
 
 
@@ -432,7 +434,8 @@ def test_autotuned_kernels_match_oracle(f32):
     scratch.zero()
     assert mgr.tune(ids, scratch) >= 1
     report = mgr.tuning_report()
-    assert len(report) == len(ids) and all(e["chosen"] in (1, 3, 4) for e in report)
+    # chosen: direct (1), wave tile (3), matrix core (4), swizzled wave tile (5)
+    assert len(report) == len(ids) and all(e["chosen"] in (1, 3, 4, 5) for e in report)
     init = np.asarray(cs.random_statevector(n, 3))
     sv = cs.HipStatevector(n, prec(f32))
     sv.upload(init)
