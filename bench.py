@@ -85,6 +85,16 @@ def run_gpu(cs, n: int, precision: str, steps: int, warmup: int, max_size: int):
         mgr.sync(False)
         log(f"  warmup step {s + 1}/{warmup} done")
     mgr.sync(False)
+    ctype = "double" if precision == "f64" else "float"
+
+    def symbol(kid):
+        """Kernel symbol as the profiler names it."""
+        k, f = info[kid].k, fam[kid]
+        if f in (3, 5):
+            low = sum(1 for q in info[kid].qubits if q < 6)
+            return f"k_tile<{ctype}, {k}, {low}>"
+        return f"{KERNEL[f]}<{ctype}, {k}>"
+
     per_kernel = defaultdict(list)
     t0 = time.perf_counter()
     for s in range(steps):
@@ -92,15 +102,15 @@ def run_gpu(cs, n: int, precision: str, steps: int, warmup: int, max_size: int):
             mgr.apply(i, sv)
         kernels, _ = mgr.sync(True)
         for kid, t in kernels:
-            per_kernel[(info[kid].k, fam[kid])].append(t)
+            per_kernel[symbol(kid)].append(t)
         log(f"  timed step {s + 1}/{steps} done")
     wall = time.perf_counter() - t0
     scalar = 8 if precision == "f64" else 4
     bytes_per_gate = 2 * (1 << n) * 2 * scalar
     n_gates = len(ids)
     gbs = n_gates * steps * bytes_per_gate / wall / 1e9
-    # dominant kernel = the (k, variant) group with the most GPU time
-    (kdom, vdom), times = max(per_kernel.items(), key=lambda kv: sum(kv[1]))
+    # dominant kernel = the kernel symbol with the most GPU time
+    sym_dom, times = max(per_kernel.items(), key=lambda kv: sum(kv[1]))
     avg = sum(times) / len(times)
     achieved = bytes_per_gate / avg / 1e9
     norm = sv.norm_squared()
@@ -119,10 +129,10 @@ def run_gpu(cs, n: int, precision: str, steps: int, warmup: int, max_size: int):
         "gbs_per_gate": gbs,
         "ms_per_step": wall / steps * 1e3,
         "gate_updates_per_s": n_gates * steps / wall,
-        "dominant": {"k": kdom, "variant": vdom, "launches": len(times), "avg_ms": avg * 1e3,
+        "dominant": {"kernel": sym_dom, "launches": len(times), "avg_ms": avg * 1e3,
                      "achieved_gbs": achieved},
-        "per_k_gbs": {f"{KERNEL[v]} k={k}": bytes_per_gate / (sum(t) / len(t)) / 1e9
-                      for (k, v), t in sorted(per_kernel.items())},
+        "per_k_gbs": {s: bytes_per_gate / (sum(t) / len(t)) / 1e9
+                      for s, t in sorted(per_kernel.items())},
         "norm_after": norm,
         "profile": src,
         "crossover_ai": prof.crossover_ai,
@@ -181,6 +191,9 @@ def main() -> int:
     ap.add_argument("--max-size", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f32", action="store_true")
+    ap.add_argument("--copy-probe-qubits", type=int, default=0,
+                    help="first copy one f64 statevector of this many qubits device-to-device "
+                         "(a known-size transfer that calibrates profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); the "
@@ -188,6 +201,18 @@ def main() -> int:
         return 2
 
     import cast_simulator_amd as cs
+
+    if args.copy_probe_qubits > 0:
+        from cast_simulator_amd import abi
+
+        nq = args.copy_probe_qubits
+        nbytes = (2 << nq) * 8
+        src, dst = abi.sv_alloc(nq, abi.F64), abi.sv_alloc(nq, abi.F64)
+        abi.memcpy_dtod_async(dst, src, nbytes)
+        abi.stream_sync(None)
+        abi.sv_free(src)
+        abi.sv_free(dst)
+        log(f"[bench] copy probe: {nbytes} bytes device-to-device")
 
     log(f"[bench] f64 {args.qubits}q circuit: warmup {args.warmup}, steps {args.steps}")
     r64 = run_gpu(cs, args.qubits, "f64", args.steps, args.warmup, args.max_size)
@@ -204,6 +229,12 @@ def main() -> int:
         log(f"[bench] cpu: {cpu['value']:.2f} GB/s ({cpu['sample']})")
 
     dom = r64["dominant"]
+    # HBM bytes per launch of the dominant kernel, from the committed PMC
+    # summary (FETCH_SIZE / WRITE_SIZE runs calibrated by a known-size copy).
+    traffic = None
+    tpath = ROOT / "profiles" / "gate_f64_30q_traffic.json"
+    if args.qubits == 30 and tpath.exists():
+        traffic = json.loads(tpath.read_text())["hbm_bytes_per_launch"].get(dom["kernel"])
     line = {
         "metric": METRIC,
         "value": round(r64["gbs_per_gate"], 2),
@@ -229,12 +260,13 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"{KERNEL[dom['variant']]} k={dom['k']} f64",
+            "kernel": dom["kernel"],
             "achieved": round(dom["achieved_gbs"], 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(dom["achieved_gbs"] / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": "profiles/gate_f64_30q_traffic.json" if traffic else None,
             "avg_launch_ms": round(dom["avg_ms"], 4),
             "launches": dom["launches"],
         },

@@ -197,18 +197,19 @@ __global__ void __launch_bounds__(256) k_tile(T *__restrict__ sv, const T *__res
 #pragma unroll
     for (int q = 0; q < G; ++q) {
       const int r = h * G + q;
-      T ar0 = T(0), ar1 = T(0), ai = T(0);
+      // Four independent accumulators: dependent FMA chains of length K.
+      T ar0 = T(0), ar1 = T(0), ai0 = T(0), ai1 = T(0);
 #pragma unroll
       for (int c = 0; c < K; ++c) {
         const T mr = mat[2 * (r * K + c)];
         const T mi = mat[2 * (r * K + c) + 1];
         ar0 = __builtin_fma(mr, vr[c], ar0);
         ar1 = __builtin_fma(mi, vi[c], ar1);
-        ai = __builtin_fma(mr, vi[c], ai);
-        ai = __builtin_fma(mi, vr[c], ai);
+        ai0 = __builtin_fma(mr, vi[c], ai0);
+        ai1 = __builtin_fma(mi, vr[c], ai1);
       }
       orr[q] = ar0 - ar1;
-      oi[q] = ai;
+      oi[q] = ai0 + ai1;
     }
 #pragma unroll
     for (int j = 0; j < L; ++j) {
