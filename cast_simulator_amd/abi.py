@@ -69,6 +69,48 @@ class DeviceInfo(C.Structure):
     ]
 
 
+class TrajGate(C.Structure):
+    _fields_ = [
+        ("matrix_offset", C.c_uint32),
+        ("k", C.c_uint8),
+        ("q", C.c_uint8 * 6),
+        ("pad", C.c_uint8),
+    ]
+
+
+class TrajTask(C.Structure):
+    _fields_ = [
+        ("first_gate", C.c_uint32),
+        ("n_gates", C.c_uint32),
+        ("checkpoint", C.c_uint32),
+        ("pad", C.c_uint32),
+    ]
+
+
+class TrajLaunch(C.Structure):
+    _fields_ = [
+        ("precision", C.c_uint32),
+        ("n_qubits", C.c_uint32),
+        ("tasks", C.c_uint64),
+        ("order", C.c_uint64),
+        ("n_tasks", C.c_uint32),
+        ("n_gates", C.c_uint32),
+        ("gates", C.c_uint64),
+        ("matrices", C.c_uint64),
+        ("n_matrix_elems", C.c_uint64),
+        ("checkpoints", C.c_uint64),
+        ("n_checkpoints", C.c_uint32),
+        ("n_slots", C.c_uint32),
+        ("slots", C.c_uint64),
+        ("block", C.c_uint32),
+        ("n_pairs", C.c_uint32),
+        ("pairs", C.c_uint32 * 64),
+        ("out", C.c_uint64),
+        ("dump", C.c_uint64),
+        ("work", C.c_uint64),
+    ]
+
+
 _vp, _sz, _u8, _u32, _u64, _i = C.c_void_p, C.c_size_t, C.c_uint8, C.c_uint32, C.c_uint64, C.c_int
 _err = C.c_char_p
 _SIGS = {
@@ -84,6 +126,10 @@ _SIGS = {
     "cast_hip_event_destroy": (None, [_vp]),
     "cast_hip_event_elapsed_ms": (_i, [_vp, _vp, C.POINTER(C.c_float), _err, _sz]),
     "cast_hip_memcpy_dtod_async": (_i, [_u64, _u64, _sz, _vp, _err, _sz]),
+    "cast_hip_malloc": (_u64, [_sz, _err, _sz]),
+    "cast_hip_free": (None, [_u64]),
+    "cast_hip_memcpy_htod_async": (_i, [_u64, _vp, _sz, _vp, _err, _sz]),
+    "cast_hip_memcpy_dtoh": (_i, [_vp, _u64, _sz, _vp, _err, _sz]),
     "cast_hip_sv_alloc": (_u64, [_sz, _u8, _err, _sz]),
     "cast_hip_sv_free": (None, [_u64]),
     "cast_hip_sv_zero": (_i, [_u64, _sz, _u8, _err, _sz]),
@@ -104,6 +150,7 @@ _SIGS = {
         _i, [C.POINTER(_vp), C.POINTER(_u64), C.POINTER(_u32), _sz, _vp, _err, _sz]),
     "cast_hip_expect_zz": (
         _i, [_u64, _u32, _u8, C.POINTER(_u32), _sz, C.POINTER(C.c_double), _vp, _err, _sz]),
+    "cast_hip_traj_run": (_i, [C.POINTER(TrajLaunch), _vp, _err, _sz]),
 }
 
 
@@ -274,6 +321,32 @@ def event_destroy(e: int) -> None:
 
 def memcpy_dtod_async(dst: int, src: int, n_bytes: int, stream: int | None = None) -> None:
     _call("cast_hip_memcpy_dtod_async", dst, src, n_bytes, stream)
+
+
+def malloc(n_bytes: int) -> int:
+    return _ctor("cast_hip_malloc", n_bytes)
+
+
+def free(dptr: int) -> None:
+    lib().cast_hip_free(dptr)
+
+
+def memcpy_htod(dst: int, host: np.ndarray, stream: int | None = None) -> None:
+    """Copy a C-contiguous host array to device memory and wait."""
+    a = np.ascontiguousarray(host)
+    _call("cast_hip_memcpy_htod_async", dst, a.ctypes.data, a.nbytes, stream)
+    stream_sync(stream)
+
+
+def memcpy_dtoh(host: np.ndarray, src: int, stream: int | None = None) -> None:
+    """Fill a C-contiguous host array from device memory (synchronous)."""
+    if not host.flags.c_contiguous:
+        raise ValueError("host array must be C-contiguous")
+    _call("cast_hip_memcpy_dtoh", host.ctypes.data, src, host.nbytes, stream)
+
+
+def traj_run(args: TrajLaunch, stream: int | None = None) -> None:
+    _call("cast_hip_traj_run", C.byref(args), stream)
 
 
 def version() -> str:

@@ -335,6 +335,45 @@ int cast_hip_memcpy_dtod_async(uint64_t dst, uint64_t src, size_t n_bytes, void 
   });
 }
 
+uint64_t cast_hip_malloc(size_t n_bytes, char *err, size_t len) {
+  void *p = nullptr;
+  const int rc = guard(err, len, [&] {
+    if (n_bytes == 0)
+      throw Error("n_bytes must be positive");
+    ensure_device();
+    check(hipMalloc(&p, n_bytes), "hipMalloc");
+  });
+  return rc == 0 ? reinterpret_cast<uint64_t>(p) : 0;
+}
+
+void cast_hip_free(uint64_t dptr) {
+  if (dptr)
+    (void)hipFree(reinterpret_cast<void *>(dptr));
+}
+
+int cast_hip_memcpy_htod_async(uint64_t dst, const void *src, size_t n_bytes, void *stream,
+                               char *err, size_t len) {
+  return guard(err, len, [&] {
+    if (!dst || (!src && n_bytes))
+      throw Error("null pointer");
+    check(hipMemcpyAsync(reinterpret_cast<void *>(dst), src, n_bytes, hipMemcpyHostToDevice,
+                         static_cast<hipStream_t>(stream)),
+          "hipMemcpyAsync(HtoD)");
+  });
+}
+
+int cast_hip_memcpy_dtoh(void *dst, uint64_t src, size_t n_bytes, void *stream, char *err,
+                         size_t len) {
+  return guard(err, len, [&] {
+    if (!src || (!dst && n_bytes))
+      throw Error("null pointer");
+    check(hipMemcpyAsync(dst, reinterpret_cast<const void *>(src), n_bytes, hipMemcpyDeviceToHost,
+                         static_cast<hipStream_t>(stream)),
+          "hipMemcpyAsync(DtoH)");
+    check(hipStreamSynchronize(static_cast<hipStream_t>(stream)), "hipStreamSynchronize");
+  });
+}
+
 uint64_t cast_hip_sv_alloc(size_t n_elements, uint8_t precision, char *err, size_t len) {
   void *p = nullptr;
   const int rc = guard(err, len, [&] {
@@ -527,6 +566,14 @@ int cast_hip_expect_zz(uint64_t sv_dptr, uint32_t sv_n_qubits, uint8_t precision
         throw Error("pair qubit " + std::to_string(pairs[p]) + " outside the statevector");
     expect_zz(reinterpret_cast<const void *>(sv_dptr), sv_n_qubits, precision, pairs, n_pairs,
               out, static_cast<hipStream_t>(stream));
+  });
+}
+
+int cast_hip_traj_run(const cast_hip_traj_launch_t *args, void *stream, char *err, size_t len) {
+  return guard(err, len, [&] {
+    if (!args)
+      throw Error("null trajectory launch descriptor");
+    launch_traj(*args, static_cast<hipStream_t>(stream));
   });
 }
 

@@ -1,6 +1,8 @@
 // Shared helpers of the gfx950 backend library (libcast_hip.so).
 #pragma once
 
+#include "cast_hip.h"
+
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
@@ -111,5 +113,9 @@ void launch_scale(void *sv, size_t n_elements, uint8_t precision, double scale,
                   hipStream_t stream);
 void expect_zz(const void *sv, uint32_t n_qubits, uint8_t precision, const uint32_t *pairs,
                size_t n_pairs, double *out, hipStream_t stream);
+
+// Persistent batched-trajectory launch (traj_kernels.hip); validates the
+// descriptor and enqueues the counter reset and the kernel.
+void launch_traj(const cast_hip_traj_launch_t &args, hipStream_t stream);
 
 } // namespace cast_hip

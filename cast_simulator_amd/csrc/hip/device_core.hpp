@@ -1,0 +1,279 @@
+// Device-side building blocks shared by the gate kernels and the persistent
+// trajectory kernel: bit-insert addressing and the wave-tile gate update.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace cast_hip {
+namespace dev {
+
+template <typename T> struct V2;
+template <> struct V2<float> {
+  using type = float2;
+};
+template <> struct V2<double> {
+  using type = double2;
+};
+
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef double d2v __attribute__((ext_vector_type(2)));
+template <typename T> struct NV;
+template <> struct NV<float> {
+  using type = f2v;
+};
+template <> struct NV<double> {
+  using type = d2v;
+};
+
+template <typename T>
+__device__ __forceinline__ typename NV<T>::type ld_stream(const T *p) {
+  return __builtin_nontemporal_load(reinterpret_cast<const typename NV<T>::type *>(p));
+}
+template <typename T>
+__device__ __forceinline__ void st_stream(T *p, typename NV<T>::type v) {
+  __builtin_nontemporal_store(v, reinterpret_cast<typename NV<T>::type *>(p));
+}
+
+// Fused multiply-add in the operand precision (__builtin_fma is the double
+// builtin: float operands would be promoted and rounded back).
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) {
+  return __builtin_fma(a, b, c);
+}
+
+// A wave-uniform value the compiler cannot see through (kept in an SGPR).
+// Adding opaque_u32(0) to a pointer keeps its provenance (so loads through it
+// stay scalar loads) while stopping the compiler from hoisting or batching
+// the loads across the point where it is taken.
+__device__ __forceinline__ uint32_t opaque_u32(uint32_t v) {
+  asm volatile("" : "+s"(v));
+  return v;
+}
+
+__device__ __forceinline__ uint64_t insert_zero(uint64_t x, uint32_t q) {
+  const uint64_t lo_mask = (uint64_t(1) << q) - 1;
+  return ((x & ~lo_mask) << 1) | (x & lo_mask);
+}
+
+// Deposit the combo index into the non-target bit positions (the
+// bit-deposit of src/cpp/cuda/cuda_gen.cpp:174-211).  Targets are ascending,
+// so inserting zeros from the lowest position up is exact.
+template <int KL>
+__device__ __forceinline__ uint64_t combo_base(uint64_t c, const uint32_t *q) {
+#pragma unroll
+  for (int b = 0; b < KL; ++b)
+    c = insert_zero(c, q[b]);
+  return c;
+}
+
+// Address offset of local basis state i: bit b of i sets bit q[b]
+// (cuda_gen.cpp:241-246).
+template <int KL>
+__device__ __forceinline__ uint64_t local_delta(uint32_t i, const uint32_t *q) {
+  uint64_t d = 0;
+#pragma unroll
+  for (int b = 0; b < KL; ++b)
+    d |= uint64_t((i >> b) & 1u) << q[b];
+  return d;
+}
+
+// Dense update of one combo: K amplitudes read from `src`, K written to
+// `dst` (src == dst for in-place), new[r] = sum_c M[r][c] old[c].
+template <typename T, int KL>
+__device__ __forceinline__ void dense_combo(const typename V2<T>::type *src,
+                                            typename V2<T>::type *dst, const T *__restrict__ mat,
+                                            uint64_t base, const uint64_t *delta) {
+  constexpr int K = 1 << KL;
+  using VT = typename V2<T>::type;
+  T vr[K], vi[K];
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    const VT x = src[base + delta[i]];
+    vr[i] = x.x;
+    vi[i] = x.y;
+  }
+#pragma unroll
+  for (int r = 0; r < K; ++r) {
+    T ar0 = T(0), ar1 = T(0), ai0 = T(0), ai1 = T(0);
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+      const T mr = mat[2 * (r * K + c)];
+      const T mi = mat[2 * (r * K + c) + 1];
+      ar0 = fma_t(mr, vr[c], ar0);
+      ar1 = fma_t(mi, vi[c], ar1);
+      ai0 = fma_t(mr, vi[c], ai0);
+      ai1 = fma_t(mi, vr[c], ai1);
+    }
+    VT out;
+    out.x = ar0 - ar1;
+    out.y = ai0 + ai1;
+    dst[base + delta[r]] = out;
+  }
+}
+
+// Wave tiles.  A wave owns 64 * K amplitudes: the 6 lowest address bits
+// (one per lane) and K register slots.  Register bit b lives at address bit
+// reg_pos[b] during load and store, so every load and store instruction moves
+// 64 consecutive amplitudes.  The L targets below bit 6 (low_t) live in lane
+// bits after the load; each is exchanged with register bit j by xor shuffles
+// so that the registers of a lane then hold the K target states of one combo.
+// Outputs are produced 2^L rows at a time, exchanged back, and stored.
+
+// U wave tiles per step (U = 1 for the one-tile-per-wave gate kernel; more
+// in loops where one workgroup must keep enough loads in flight on its own).
+// src[u] / dst[u] point at scalar 2 * (tile_base | lane) of tile u (equal for
+// in place).  All loads of the step are issued before any store.  Rows are the
+// outer loop and tiles the inner one, so one matrix row (2 K scalars, read by
+// scalar loads) is live at a time whatever U is.  Register arrays are local
+// and indexed by constants only, so they stay in VGPRs.
+template <typename T, int KL, int L, int U>
+__device__ __forceinline__ void tile_step(const T *const *src, T *const *dst,
+                                          const T *__restrict__ mat, const uint32_t *reg_pos,
+                                          const uint32_t *low_t, uint32_t lane) {
+  constexpr int K = 1 << KL, G = 1 << L;
+  T vr[U][K], vi[U][K];
+#pragma unroll
+  for (int u = 0; u < U; ++u)
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      const auto x = ld_stream(src[u] + 2 * local_delta<KL>(uint32_t(i), reg_pos));
+      vr[u][i] = x.x;
+      vi[u][i] = x.y;
+    }
+  // lane bit low_t[j]  <->  register bit j
+#pragma unroll
+  for (int j = 0; j < L; ++j) {
+    const int m = 1 << low_t[j];
+    const bool up = (lane >> low_t[j]) & 1u;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int r = 0; r < K; ++r) {
+        if (r & (1 << j))
+          continue;
+        const int r1 = r | (1 << j);
+        const T sr = up ? vr[u][r] : vr[u][r1], si = up ? vi[u][r] : vi[u][r1];
+        const T gr = __shfl_xor(sr, m, 64), gi = __shfl_xor(si, m, 64);
+        if (up) {
+          vr[u][r] = gr;
+          vi[u][r] = gi;
+        } else {
+          vr[u][r1] = gr;
+          vi[u][r1] = gi;
+        }
+      }
+  }
+#pragma unroll
+  for (int h = 0; h < K / G; ++h) {
+    T orr[U][G], oi[U][G];
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+      const int r = h * G + q;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        // Four independent accumulators: dependent FMA chains of length K.
+        T ar0 = T(0), ar1 = T(0), ai0 = T(0), ai1 = T(0);
+#pragma unroll
+        for (int c = 0; c < K; ++c) {
+          const T mr = mat[2 * (r * K + c)];
+          const T mi = mat[2 * (r * K + c) + 1];
+          ar0 = fma_t(mr, vr[u][c], ar0);
+          ar1 = fma_t(mi, vi[u][c], ar1);
+          ai0 = fma_t(mr, vi[u][c], ai0);
+          ai1 = fma_t(mi, vr[u][c], ai1);
+        }
+        orr[u][q] = ar0 - ar1;
+        oi[u][q] = ai0 + ai1;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+      const int m = 1 << low_t[j];
+      const bool up = (lane >> low_t[j]) & 1u;
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+          if (q & (1 << j))
+            continue;
+          const int q1 = q | (1 << j);
+          const T sr = up ? orr[u][q] : orr[u][q1], si = up ? oi[u][q] : oi[u][q1];
+          const T gr = __shfl_xor(sr, m, 64), gi = __shfl_xor(si, m, 64);
+          if (up) {
+            orr[u][q] = gr;
+            oi[u][q] = gi;
+          } else {
+            orr[u][q1] = gr;
+            oi[u][q1] = gi;
+          }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int q = 0; q < G; ++q) {
+        typename NV<T>::type o;
+        o.x = orr[u][q];
+        o.y = oi[u][q];
+        st_stream(dst[u] + 2 * local_delta<KL>(uint32_t(h * G + q), reg_pos), o);
+      }
+  }
+}
+
+// One wave-tile update (src == dst for in place).
+template <typename T, int KL, int L>
+__device__ __forceinline__ void tile_update(const T *src, T *dst, const T *__restrict__ mat,
+                                            const uint32_t *reg_pos, const uint32_t *low_t,
+                                            uint32_t lane) {
+  const T *const s[1] = {src};
+  T *const d[1] = {dst};
+  tile_step<T, KL, L, 1>(s, d, mat, reg_pos, low_t, lane);
+}
+
+// Wave-tile geometry of a gate with ascending targets q[0..k) on an n-qubit
+// statevector (n >= k + 6).  F = the first L non-target positions >= 6.
+struct TileLayout {
+  uint32_t ins[6];     // ascending insert positions (minus 6) of F and targets >= 6
+  uint32_t reg_pos[6]; // address bit of register bit b during load/store
+  uint32_t low_t[6];   // targets below 6
+  uint32_t n_low;
+};
+
+__host__ __device__ inline TileLayout tile_layout(const uint32_t *q, uint32_t k) {
+  TileLayout t{};
+  uint32_t L = 0, h = 0, high[6] = {};
+  for (uint32_t b = 0; b < k; ++b) {
+    if (q[b] < 6)
+      t.low_t[L++] = q[b];
+    else
+      high[h++] = q[b];
+  }
+  uint32_t nf = 0;
+  for (uint32_t pos = 6; nf < L; ++pos) {
+    bool is_t = false;
+    for (uint32_t b = 0; b < k; ++b)
+      is_t = is_t || q[b] == pos;
+    if (!is_t)
+      t.reg_pos[nf++] = pos;
+  }
+  for (uint32_t j = 0; j < h; ++j)
+    t.reg_pos[L + j] = high[j];
+  uint32_t ins[6];
+  for (uint32_t b = 0; b < k; ++b)
+    ins[b] = t.reg_pos[b];
+  for (uint32_t a = 1; a < k; ++a) // insertion sort (k <= 6)
+    for (uint32_t b = a; b > 0 && ins[b - 1] > ins[b]; --b) {
+      const uint32_t x = ins[b];
+      ins[b] = ins[b - 1];
+      ins[b - 1] = x;
+    }
+  for (uint32_t b = 0; b < k; ++b)
+    t.ins[b] = ins[b] - 6;
+  t.n_low = L;
+  return t;
+}
+
+} // namespace dev
+} // namespace cast_hip

@@ -23,6 +23,7 @@
 
 #include "cast_hip.h"
 #include "common.hpp"
+#include "device_core.hpp"
 
 #include <algorithm>
 #include <cstdlib>
@@ -30,67 +31,15 @@
 namespace cast_hip {
 namespace {
 
-template <typename T> struct V2;
-template <> struct V2<float> { using type = float2; };
-template <> struct V2<double> { using type = double2; };
-
-__device__ __forceinline__ uint64_t insert_zero(uint64_t x, uint32_t q) {
-  const uint64_t lo_mask = (uint64_t(1) << q) - 1;
-  return ((x & ~lo_mask) << 1) | (x & lo_mask);
-}
-
-// Deposit the combo index into the non-target bit positions.  Targets are
-// ascending, so inserting zeros from the lowest position up is exact.
-template <int KL>
-__device__ __forceinline__ uint64_t combo_base(uint64_t c, const uint32_t *q) {
-#pragma unroll
-  for (int b = 0; b < KL; ++b)
-    c = insert_zero(c, q[b]);
-  return c;
-}
-
-template <int KL>
-__device__ __forceinline__ uint64_t local_delta(uint32_t i, const uint32_t *q) {
-  uint64_t d = 0;
-#pragma unroll
-  for (int b = 0; b < KL; ++b)
-    d |= uint64_t((i >> b) & 1u) << q[b];
-  return d;
-}
+using dev::combo_base;
+using dev::insert_zero;
+using dev::ld_stream;
+using dev::local_delta;
+using dev::NV;
+using dev::st_stream;
+using dev::V2;
 
 // ── Dense: one combo per lane ───────────────────────────────────────────────
-
-template <typename T, int KL>
-__device__ __forceinline__ void dense_update(typename V2<T>::type *__restrict__ amp,
-                                             const T *__restrict__ mat, uint64_t base,
-                                             const uint64_t *delta) {
-  constexpr int K = 1 << KL;
-  using VT = typename V2<T>::type;
-  T vr[K], vi[K];
-#pragma unroll
-  for (int i = 0; i < K; ++i) {
-    const VT x = amp[base + delta[i]];
-    vr[i] = x.x;
-    vi[i] = x.y;
-  }
-#pragma unroll
-  for (int r = 0; r < K; ++r) {
-    T ar0 = T(0), ar1 = T(0), ai = T(0);
-#pragma unroll
-    for (int c = 0; c < K; ++c) {
-      const T mr = mat[2 * (r * K + c)];
-      const T mi = mat[2 * (r * K + c) + 1];
-      ar0 = __builtin_fma(mr, vr[c], ar0);
-      ar1 = __builtin_fma(mi, vi[c], ar1);
-      ai = __builtin_fma(mr, vi[c], ai);
-      ai = __builtin_fma(mi, vr[c], ai);
-    }
-    VT out;
-    out.x = ar0 - ar1;
-    out.y = ai;
-    amp[base + delta[r]] = out;
-  }
-}
 
 template <typename T, int KL>
 __global__ void __launch_bounds__(256) k_dense(T *__restrict__ sv, const T *__restrict__ mat,
@@ -106,7 +55,7 @@ __global__ void __launch_bounds__(256) k_dense(T *__restrict__ sv, const T *__re
   // per wave through scalar loads instead of being hoisted into VGPRs.
   const uint64_t c = (uint64_t(blockIdx.y) * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x;
   if (c < g.combos)
-    dense_update<T, KL>(amp, mat, combo_base<KL>(c, g.q), delta);
+    dev::dense_combo<T, KL>(amp, amp, mat, combo_base<KL>(c, g.q), delta);
 }
 
 // ── Wave tiles with cross-lane exchange of low-order targets ───────────────
@@ -121,21 +70,6 @@ __global__ void __launch_bounds__(256) k_dense(T *__restrict__ sv, const T *__re
 // register bits then enumerate exactly the K target states of one combo per
 // lane.  Outputs are produced 2^L rows at a time, exchanged back, and stored.
 
-typedef float f2v __attribute__((ext_vector_type(2)));
-typedef double d2v __attribute__((ext_vector_type(2)));
-template <typename T> struct NV;
-template <> struct NV<float> { using type = f2v; };
-template <> struct NV<double> { using type = d2v; };
-
-template <typename T>
-__device__ __forceinline__ typename NV<T>::type ld_stream(const T *p) {
-  return __builtin_nontemporal_load(reinterpret_cast<const typename NV<T>::type *>(p));
-}
-template <typename T>
-__device__ __forceinline__ void st_stream(T *p, typename NV<T>::type v) {
-  __builtin_nontemporal_store(v, reinterpret_cast<typename NV<T>::type *>(p));
-}
-
 struct TileGeom {
   uint64_t n_tiles;    // 2^(n - 6 - k)
   uint32_t ins[6];     // ascending insert positions (minus 6) of F and targets >= 6
@@ -147,7 +81,6 @@ struct TileGeom {
 template <typename T, int KL, int L>
 __global__ void __launch_bounds__(256) k_tile(T *__restrict__ sv, const T *__restrict__ mat,
                                               TileGeom g) {
-  constexpr int K = 1 << KL, G = 1 << L;
   const uint32_t lane = threadIdx.x & 63u;
   uint64_t tile =
       ((uint64_t(blockIdx.y) * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
@@ -163,82 +96,7 @@ __global__ void __launch_bounds__(256) k_tile(T *__restrict__ sv, const T *__res
     tb = insert_zero(tb, g.ins[b]);
   T *__restrict__ p = sv + 2 * ((tb << 6) | lane);
 
-  T vr[K], vi[K];
-#pragma unroll
-  for (int i = 0; i < K; ++i) {
-    const auto x = ld_stream(p + 2 * local_delta<KL>(uint32_t(i), g.reg_pos));
-    vr[i] = x.x;
-    vi[i] = x.y;
-  }
-  // lane bit low_t[j]  <->  register bit j
-#pragma unroll
-  for (int j = 0; j < L; ++j) {
-    const int m = 1 << g.low_t[j];
-    const bool up = (lane >> g.low_t[j]) & 1u;
-#pragma unroll
-    for (int r = 0; r < K; ++r) {
-      if (r & (1 << j))
-        continue;
-      const int r1 = r | (1 << j);
-      const T sr = up ? vr[r] : vr[r1], si = up ? vi[r] : vi[r1];
-      const T gr = __shfl_xor(sr, m, 64), gi = __shfl_xor(si, m, 64);
-      if (up) {
-        vr[r] = gr;
-        vi[r] = gi;
-      } else {
-        vr[r1] = gr;
-        vi[r1] = gi;
-      }
-    }
-  }
-#pragma unroll
-  for (int h = 0; h < K / G; ++h) {
-    T orr[G], oi[G];
-#pragma unroll
-    for (int q = 0; q < G; ++q) {
-      const int r = h * G + q;
-      // Four independent accumulators: dependent FMA chains of length K.
-      T ar0 = T(0), ar1 = T(0), ai0 = T(0), ai1 = T(0);
-#pragma unroll
-      for (int c = 0; c < K; ++c) {
-        const T mr = mat[2 * (r * K + c)];
-        const T mi = mat[2 * (r * K + c) + 1];
-        ar0 = __builtin_fma(mr, vr[c], ar0);
-        ar1 = __builtin_fma(mi, vi[c], ar1);
-        ai0 = __builtin_fma(mr, vi[c], ai0);
-        ai1 = __builtin_fma(mi, vr[c], ai1);
-      }
-      orr[q] = ar0 - ar1;
-      oi[q] = ai0 + ai1;
-    }
-#pragma unroll
-    for (int j = 0; j < L; ++j) {
-      const int m = 1 << g.low_t[j];
-      const bool up = (lane >> g.low_t[j]) & 1u;
-#pragma unroll
-      for (int q = 0; q < G; ++q) {
-        if (q & (1 << j))
-          continue;
-        const int q1 = q | (1 << j);
-        const T sr = up ? orr[q] : orr[q1], si = up ? oi[q] : oi[q1];
-        const T gr = __shfl_xor(sr, m, 64), gi = __shfl_xor(si, m, 64);
-        if (up) {
-          orr[q] = gr;
-          oi[q] = gi;
-        } else {
-          orr[q1] = gr;
-          oi[q1] = gi;
-        }
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < G; ++q) {
-      typename NV<T>::type o;
-      o.x = orr[q];
-      o.y = oi[q];
-      st_stream(p + 2 * local_delta<KL>(uint32_t(h * G + q), g.reg_pos), o);
-    }
-  }
+  dev::tile_update<T, KL, L>(p, p, mat, g.reg_pos, g.low_t, lane);
 }
 
 // ── Monomial: at most one nonzero per row ───────────────────────────────────
@@ -392,7 +250,7 @@ __global__ void __launch_bounds__(256) k_dense_batch(BatchArgs a) {
     delta[i] = local_delta<KL>(uint32_t(i), q);
   const uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (c < it.combos)
-    dense_update<T, KL>(amp, mat, combo_base<KL>(c, q), delta);
+    dev::dense_combo<T, KL>(amp, amp, mat, combo_base<KL>(c, q), delta);
 }
 
 // ── Launch helpers ──────────────────────────────────────────────────────────
@@ -504,32 +362,13 @@ void launch_tile_k(const GatePlan &p, hipStream_t s, void *sv, const TileGeom &t
 TileGeom tile_geom(const GatePlan &p, uint32_t n, uint32_t *n_low) {
   TileGeom tg{};
   tg.n_tiles = uint64_t(1) << (n - 6 - p.k);
-  uint32_t L = 0, h = 0, high[6] = {};
-  for (uint32_t b = 0; b < p.k; ++b) {
-    if (p.q[b] < 6)
-      tg.low_t[L++] = p.q[b];
-    else
-      high[h++] = p.q[b];
+  const dev::TileLayout t = dev::tile_layout(p.q, p.k);
+  for (int b = 0; b < 6; ++b) {
+    tg.ins[b] = t.ins[b];
+    tg.reg_pos[b] = t.reg_pos[b];
+    tg.low_t[b] = t.low_t[b];
   }
-  // F: the first L non-target positions >= 6.
-  uint32_t f[6] = {}, nf = 0;
-  for (uint32_t pos = 6; nf < L; ++pos) {
-    bool is_t = false;
-    for (uint32_t b = 0; b < p.k; ++b)
-      is_t = is_t || p.q[b] == pos;
-    if (!is_t)
-      f[nf++] = pos;
-  }
-  for (uint32_t j = 0; j < L; ++j)
-    tg.reg_pos[j] = f[j];
-  for (uint32_t j = 0; j < h; ++j)
-    tg.reg_pos[L + j] = high[j];
-  uint32_t ins[6];
-  for (uint32_t b = 0; b < p.k; ++b)
-    ins[b] = tg.reg_pos[b];
-  std::sort(ins, ins + p.k);
-  for (uint32_t b = 0; b < p.k; ++b)
-    tg.ins[b] = ins[b] - 6;
+  const uint32_t L = t.n_low;
   if (p.flags & CAST_HIP_FLAG_TILE_SWIZZLE)
     tg.swz = std::min<uint32_t>(10, n - 6 - p.k);
   *n_low = L;

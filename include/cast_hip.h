@@ -138,6 +138,20 @@ int cast_hip_event_elapsed_ms(void *start_event, void *end_event, float *out_ms,
 int cast_hip_memcpy_dtod_async(uint64_t dst, uint64_t src, size_t n_bytes, void *stream,
                                char *err_buf, size_t err_buf_len);
 
+/* ── Raw device memory (no reference counterpart beyond cuMemAlloc inside
+ *    cast_cuda_sv_alloc, cuda_exec.cpp:429-449) — for host engines that lay
+ *    out task tables and matrix pools in device memory. ───────────────────── */
+
+uint64_t cast_hip_malloc(size_t n_bytes, char *err_buf, size_t err_buf_len); /* 0 on failure */
+void cast_hip_free(uint64_t dptr);
+/* Host -> device copy on `stream`; the host buffer must stay valid until the
+ * stream reaches the copy (synchronise before reusing it). */
+int cast_hip_memcpy_htod_async(uint64_t dst, const void *src, size_t n_bytes, void *stream,
+                               char *err_buf, size_t err_buf_len);
+/* Device -> host copy; returns after completion. */
+int cast_hip_memcpy_dtoh(void *dst, uint64_t src, size_t n_bytes, void *stream, char *err_buf,
+                         size_t err_buf_len);
+
 /* ── Statevector memory (ffi_cuda.h:80-88) ───────────────────────────────── */
 
 /* Returns 0 on failure (e.g. out of device memory). */
@@ -206,6 +220,60 @@ int cast_hip_gate_apply_batch(const void *const *plans, const uint64_t *svs,
 int cast_hip_expect_zz(uint64_t sv_dptr, uint32_t sv_n_qubits, uint8_t precision,
                        const uint32_t *pairs, size_t n_pairs, double *out, void *stream,
                        char *err_buf, size_t err_buf_len);
+
+/* ── Batched trajectories in one persistent launch ───────────────────────────
+ * No reference counterpart (the reference branches deterministic ensembles
+ * on the host, src/simulator/trajectory.rs:180-265).  Each workgroup owns
+ * one statevector slot and takes whole tasks from an atomic counter until the
+ * task list is exhausted (no workgroup ever waits on another).  A task starts
+ * from a checkpoint statevector, applies its gate list (k <= 4) — the first
+ * gate reads the checkpoint and writes the slot — and writes f64 sums of
+ * |amp|^2 and of |amp|^2 (-1)^(bit a xor bit b) for each requested pair. */
+typedef struct cast_hip_traj_gate_t {
+  uint32_t matrix_offset; /* complex elements into the matrix pool      */
+  uint8_t k;              /* 1..4                                       */
+  uint8_t q[6];           /* ascending target qubits                    */
+  uint8_t pad;
+} cast_hip_traj_gate_t;
+
+typedef struct cast_hip_traj_task_t {
+  uint32_t first_gate; /* into the gate table                        */
+  uint32_t n_gates;
+  uint32_t checkpoint; /* into the checkpoint table                  */
+  uint32_t pad;
+} cast_hip_traj_task_t;
+
+typedef struct cast_hip_traj_launch_t {
+  uint32_t precision;     /* 0 = F32, 1 = F64                           */
+  uint32_t n_qubits;
+  uint64_t tasks;         /* device cast_hip_traj_task_t[n_tasks]         */
+  uint64_t order;         /* device uint32_t[n_tasks]: task visit order   */
+  uint32_t n_tasks;
+  uint32_t n_gates;       /* size of the gate table                     */
+  uint64_t gates;         /* device cast_hip_traj_gate_t[n_gates]         */
+  uint64_t matrices;      /* device complex pool (run precision)        */
+  uint64_t n_matrix_elems;/* complex elements in the pool               */
+  uint64_t checkpoints;   /* device: n_checkpoints statevectors of 2^n  */
+                          /* amplitudes each, contiguous                */
+  uint32_t n_checkpoints;
+  uint32_t n_slots;       /* workgroups = statevector slots             */
+  uint64_t slots;         /* device: n_slots * 2^n amplitudes           */
+  uint32_t block;         /* threads per workgroup: 64, 128, 256 or 512 */
+  uint32_t n_pairs;       /* <= 32                                      */
+  uint32_t pairs[64];     /* a0, b0, a1, b1, ...                        */
+  uint64_t out;           /* device double[n_tasks * (1 + n_pairs)]     */
+  uint64_t dump;          /* 0, or device buffer for final states:      */
+                          /* n_tasks * 2^n amplitudes, indexed by task  */
+  uint64_t work;          /* device scratch: 2 * uint32_t (counter, error flag) */
+} cast_hip_traj_launch_t;
+
+/* Enqueue the persistent launch on `stream` (returns without waiting).  A
+ * gate descriptor that is invalid on the device (k outside 1..4, a qubit
+ * >= n, a matrix outside the pool, a checkpoint outside the table) is
+ * skipped and sets the error flag (second uint32_t of `work`), which the
+ * caller checks after the stream completes. */
+int cast_hip_traj_run(const cast_hip_traj_launch_t *args, void *stream, char *err_buf,
+                      size_t err_buf_len);
 
 /* Library version string (static storage). */
 const char *cast_hip_version(void);
