@@ -40,12 +40,18 @@ from ._host import (  # noqa: E402,F401
     HipKernelGenSpec,
     HipKernelManager,
     HipStatevector,
+    LayeredCircuit,
+    NoiseSpec,
     Precision,
     ProfileConfig,
     QuantumCircuit,
     QuantumGate,
     Simulator,
     SizeOnlyCostModel,
+    TrajectoryEngine,
+    TrajectoryRun,
+    TrajectorySample,
+    TrajectoryTuning,
     adaptive_sweep,
     apply_size_two_fusion,
     compute_bw_slope,
@@ -59,7 +65,9 @@ from ._host import (  # noqa: E402,F401
     random_layer_circuit,
     random_statevector,
     random_unitary_matrix,
+    sample_trajectories,
     superoperator_of,
+    trajectory_circuit,
 )
 
 LIB_PATH = str(_LIB)

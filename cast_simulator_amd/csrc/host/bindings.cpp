@@ -12,6 +12,9 @@
 #include "cost_model.hpp"
 #include "fusion.hpp"
 #include "hip_backend.hpp"
+#include "trajectory.hpp"
+
+#include <cstring>
 #include "profile.hpp"
 #include "types.hpp"
 
@@ -130,6 +133,8 @@ PYBIND11_MODULE(_host, m) {
       .def("arithmetic_intensity", &QuantumGate::arithmetic_intensity)
       .def("matmul", &QuantumGate::matmul)
       .def("to_density_matrix_gate", &QuantumGate::to_density_matrix_gate)
+      .def("origin", &QuantumGate::origin)
+      .def("with_origin", &QuantumGate::with_origin)
       .def("__eq__", &QuantumGate::operator==)
       .def("__repr__", [](const QuantumGate &g) {
         std::string s = "QuantumGate(k=" + std::to_string(g.n_qubits()) + ", qubits=[";
@@ -423,4 +428,153 @@ PYBIND11_MODULE(_host, m) {
                          });
         },
         py::arg("spec"), py::arg("n_qubits"), py::arg("budget_s"), py::arg("seed") = 1);
+
+  // ── batched trajectories ───────────────────────────────────────────────
+  py::class_<LayeredCircuit>(m, "LayeredCircuit")
+      .def_static("random_layers", &LayeredCircuit::random_layers, py::arg("n"),
+                  py::arg("depth"), py::arg("seed"))
+      .def_readonly("n_qubits", &LayeredCircuit::n_qubits)
+      .def_readonly("layer_start", &LayeredCircuit::layer_start)
+      .def("n_layers", &LayeredCircuit::n_layers)
+      .def("gates", [](const LayeredCircuit &c) {
+        std::vector<std::shared_ptr<QuantumGate>> out;
+        for (const auto &g : c.gates)
+          out.push_back(std::make_shared<QuantumGate>(*g));
+        return out;
+      });
+
+  py::class_<NoiseSpec>(m, "NoiseSpec")
+      .def(py::init([](double p, uint32_t min_depth, uint32_t max_depth, uint64_t seed) {
+             NoiseSpec n;
+             n.p = p;
+             n.min_depth = min_depth;
+             n.max_depth = max_depth;
+             n.seed = seed;
+             return n;
+           }),
+           py::arg("p"), py::arg("min_depth"), py::arg("max_depth"), py::arg("seed"))
+      .def_readwrite("p", &NoiseSpec::p)
+      .def_readwrite("min_depth", &NoiseSpec::min_depth)
+      .def_readwrite("max_depth", &NoiseSpec::max_depth)
+      .def_readwrite("seed", &NoiseSpec::seed);
+
+  py::class_<TrajectorySample>(m, "TrajectorySample")
+      .def(py::init([](uint32_t depth, const std::vector<std::tuple<uint32_t, uint32_t, uint8_t>> &e) {
+             TrajectorySample s;
+             s.depth = depth;
+             for (const auto &[g, q, p] : e)
+               s.errors.push_back({g, q, p});
+             return s;
+           }),
+           py::arg("depth"), py::arg("errors"))
+      .def_readonly("depth", &TrajectorySample::depth)
+      .def_property_readonly("errors", [](const TrajectorySample &s) {
+        std::vector<std::tuple<uint32_t, uint32_t, uint8_t>> out;
+        for (const auto &e : s.errors)
+          out.emplace_back(e.gate, e.qubit, e.pauli);
+        return out;
+      });
+
+  m.def("sample_trajectories", &sample_trajectories, py::arg("circuit"), py::arg("noise"),
+        py::arg("count"), py::arg("first") = 0);
+  m.def("trajectory_circuit", &trajectory_circuit);
+
+  py::class_<TrajectoryTuning>(m, "TrajectoryTuning")
+      .def(py::init([](uint32_t chunk_layers, uint32_t fusion_max, uint32_t block,
+                       uint32_t slots_per_cu, uint32_t max_slots) {
+             TrajectoryTuning t;
+             t.chunk_layers = chunk_layers;
+             t.fusion_max = fusion_max;
+             t.block = block;
+             t.slots_per_cu = slots_per_cu;
+             t.max_slots = max_slots;
+             return t;
+           }),
+           py::arg("chunk_layers") = 4, py::arg("fusion_max") = 4, py::arg("block") = 256,
+           py::arg("slots_per_cu") = 2, py::arg("max_slots") = 0)
+      .def_readwrite("chunk_layers", &TrajectoryTuning::chunk_layers)
+      .def_readwrite("fusion_max", &TrajectoryTuning::fusion_max)
+      .def_readwrite("block", &TrajectoryTuning::block)
+      .def_readwrite("slots_per_cu", &TrajectoryTuning::slots_per_cu)
+      .def_readwrite("max_slots", &TrajectoryTuning::max_slots)
+      .def("__repr__", [](const TrajectoryTuning &t) {
+        return "TrajectoryTuning(chunk_layers=" + std::to_string(t.chunk_layers) +
+               ", fusion_max=" + std::to_string(t.fusion_max) +
+               ", block=" + std::to_string(t.block) +
+               ", slots_per_cu=" + std::to_string(t.slots_per_cu) +
+               ", max_slots=" + std::to_string(t.max_slots) + ")";
+      });
+
+  py::class_<TrajectoryRun>(m, "TrajectoryRun")
+      .def_readonly("n_pairs", &TrajectoryRun::n_pairs)
+      .def_readonly("task_of", &TrajectoryRun::task_of)
+      .def_readonly("task_ckpt_layer", &TrajectoryRun::task_ckpt_layer)
+      .def_readonly("task_gates", &TrajectoryRun::task_gates)
+      .def_readonly("norm_sq", &TrajectoryRun::norm_sq)
+      .def_readonly("zz_sum", &TrajectoryRun::zz_sum)
+      .def_readonly("traj_zz", &TrajectoryRun::traj_zz)
+      .def_readonly("mean_zz", &TrajectoryRun::mean_zz)
+      .def_readonly("min_norm_sq", &TrajectoryRun::min_norm_sq)
+      .def_readonly("max_norm_sq", &TrajectoryRun::max_norm_sq)
+      .def_readonly("plan_s", &TrajectoryRun::plan_s)
+      .def_readonly("checkpoint_s", &TrajectoryRun::checkpoint_s)
+      .def_readonly("gpu_s", &TrajectoryRun::gpu_s)
+      .def_readonly("total_s", &TrajectoryRun::total_s)
+      .def_readonly("n_tasks", &TrajectoryRun::n_tasks)
+      .def_readonly("n_gate_passes", &TrajectoryRun::n_gate_passes)
+      .def_readonly("n_shared_clean", &TrajectoryRun::n_shared_clean)
+      .def_readonly("n_dirty_matrices", &TrajectoryRun::n_dirty_matrices)
+      .def_readonly("n_slots", &TrajectoryRun::n_slots)
+      .def_readonly("block", &TrajectoryRun::block)
+      .def_readonly("chunk_layers", &TrajectoryRun::chunk_layers)
+      .def_readonly("fusion_max", &TrajectoryRun::fusion_max)
+      .def("dump", [](const TrajectoryRun &r) {
+        return py::array_t<double>(py::ssize_t(r.dump.size()), r.dump.data());
+      });
+
+  py::class_<TrajectoryEngine>(m, "TrajectoryEngine")
+      .def(py::init([](const LayeredCircuit &c, cast_precision_t prec,
+                       std::vector<uint32_t> pairs) {
+             return std::make_unique<TrajectoryEngine>(c, prec, std::move(pairs));
+           }),
+           py::arg("circuit"), py::arg("precision"), py::arg("pairs"))
+      .def("set_initial_state", &TrajectoryEngine::set_initial_state)
+      .def("plan",
+           [](TrajectoryEngine &e, const std::vector<TrajectorySample> &samples,
+              const TrajectoryTuning &t) {
+             const TrajectoryPlan P = e.plan(samples, t);
+             py::list tasks, gates;
+             for (const auto &k : P.tasks)
+               tasks.append(py::make_tuple(k.first_gate, k.n_gates, k.checkpoint));
+             for (const auto &g : P.gates) {
+               std::vector<uint32_t> q(g.q, g.q + g.k);
+               gates.append(py::make_tuple(g.matrix_offset, q));
+             }
+             py::array_t<cplx> pool(py::ssize_t(P.pool.size() / 2));
+             std::memcpy(pool.mutable_data(), P.pool.data(), P.pool.size() * sizeof(double));
+             return py::dict(py::arg("tasks") = tasks, py::arg("gates") = gates,
+                             py::arg("pool") = pool, py::arg("order") = P.order,
+                             py::arg("task_of") = P.task_of,
+                             py::arg("n_shared_clean") = P.n_shared_clean,
+                             py::arg("n_dirty_matrices") = P.n_dirty_matrices,
+                             py::arg("plan_s") = P.plan_s);
+           },
+           py::arg("samples"), py::arg("tuning"))
+      .def("run", &TrajectoryEngine::run, py::arg("samples"), py::arg("tuning"),
+           py::arg("dump") = false, py::call_guard<py::gil_scoped_release>())
+      .def("autotune",
+           [](TrajectoryEngine &e, const std::vector<TrajectorySample> &sample,
+              const TrajectoryTuning &start) {
+             std::vector<TrajectoryTuneRecord> log;
+             TrajectoryTuning best;
+             {
+               py::gil_scoped_release rel;
+               best = e.autotune(sample, start, &log);
+             }
+             py::list trials;
+             for (const auto &r : log)
+               trials.append(py::make_tuple(r.tuning, r.s_per_trajectory));
+             return py::make_tuple(best, trials);
+           },
+           py::arg("sample"), py::arg("start"));
 }

@@ -480,6 +480,10 @@ QuantumGate QuantumGate::matmul(const QuantumGate &other) const {
   const auto lhs = expand_matrix(m_, q_, uq);
   const auto rhs = expand_matrix(other.m_, other.q_, uq);
   QuantumGate g(lhs.matmul(rhs), uq);
+  if (!origin_.empty() || !other.origin_.empty()) {
+    g.origin_ = other.origin_;
+    g.origin_.insert(g.origin_.end(), origin_.begin(), origin_.end());
+  }
   if (noise_ || other.noise_) {
     auto nm = std::make_shared<NoiseModel>();
     if (noise_ && !other.noise_) {

@@ -138,6 +138,16 @@ public:
   // Superoperator lift for density-matrix mode (gate.rs:236-258).
   QuantumGate to_density_matrix_gate(size_t n_total) const;
 
+  // Provenance: ids of the source gates this gate is the product of, in
+  // application order.  Empty unless set by the caller; matmul concatenates
+  // (other's ids, then this gate's), so a fused gate lists its components.
+  const std::vector<uint32_t> &origin() const { return origin_; }
+  QuantumGate with_origin(std::vector<uint32_t> ids) const {
+    QuantumGate g = *this;
+    g.origin_ = std::move(ids);
+    return g;
+  }
+
   bool operator==(const QuantumGate &o) const;
 
 private:
@@ -145,6 +155,7 @@ private:
   ComplexSquareMatrix m_;
   std::vector<uint32_t> q_;
   std::shared_ptr<const NoiseModel> noise_;
+  std::vector<uint32_t> origin_;
 };
 
 // gate.rs:436-468: expand `m` acting on `gate_qubits` to act on `union_qubits`.
