@@ -139,6 +139,92 @@ def run_gpu(cs, n: int, precision: str, steps: int, warmup: int, max_size: int):
     }
 
 
+def run_trajectories(cs, n: int, count: int, tune_sample: int, plain_sample: int):
+    """Batched noisy trajectories (one persistent launch) against the plain
+    per-circuit f64 GPU path on the same trajectories.
+
+    Workload: the n-qubit random-layer circuit of 40 layers (seed SEED); each
+    trajectory runs a depth drawn uniformly from [10, 40] and gets a
+    depolarizing Pauli with p = 1e-3 after every gate on each touched qubit;
+    f32 amplitudes, f64 norms and <Z_i Z_i+1> sums.  circuits/s counts the
+    whole batch: host planning, clean checkpoints and the launch."""
+    import numpy as np
+
+    D, p = 40, 1e-3
+    lc = cs.LayeredCircuit.random_layers(n, D, SEED)
+    pairs = [q for i in range(n - 1) for q in (i, i + 1)]
+    noise = cs.NoiseSpec(p, 10, D, SEED)
+    S = cs.sample_trajectories(lc, noise, count)
+    n_err = sum(len(s.errors) for s in S)
+    eng = cs.TrajectoryEngine(lc, cs.Precision.F32, pairs)
+    t0 = time.perf_counter()
+    best, trials = eng.autotune(S[:tune_sample], cs.TrajectoryTuning(4, 4, 256, 2, 0))
+    t_tune = time.perf_counter() - t0
+    log(f"  trajectory autotune: {len(trials)} trials on {tune_sample} trajectories, "
+        f"{t_tune:.1f} s -> {best!r}")
+    r = eng.run(S, best)
+    sv_bytes = (2 << n) * 4
+    # algorithmic bytes: each fused gate reads and writes one statevector,
+    # each task's observation reads it once more
+    alg = r.n_gate_passes * 2 * sv_bytes + r.n_tasks * sv_bytes
+    zz = np.array(r.traj_zz)
+    log(f"  trajectories: {count} in {r.total_s:.2f} s ({count / r.total_s:.1f} circuits/s), "
+        f"gpu {r.gpu_s:.2f} s, {r.n_tasks} tasks, {r.n_gate_passes} gate passes")
+
+    # Plain per-circuit f64 GPU path on the first trajectories: explicit gate
+    # list (Paulis inserted), fused to k <= 5 by the hardware-adaptive model,
+    # one statevector each.
+    prof, _ = hw_profile(cs, "f64")
+    sim = cs.Simulator.f64()
+    sim.manager().set_autotune(False)
+    t_plain = 0.0
+    fids = []
+    sub = S[:plain_sample]
+    rd = eng.run(sub, best, True)
+    dump = rd.dump().reshape(rd.n_tasks, -1)
+    for t, s in enumerate(sub):
+        tp = time.perf_counter()
+        g = cs.CircuitGraph.from_circuit(cs.trajectory_circuit(lc, s))
+        cs.optimize(g, cs.FusionConfig.hardware_adaptive(prof, 5, 1e-12))
+        sv = sim.simulate(g)
+        t_plain += time.perf_counter() - tp
+        x = np.asarray(sv.download())
+        y = dump[rd.task_of[t]]
+        zx = x[0::2] + 1j * x[1::2]
+        zy = y[0::2] + 1j * y[1::2]
+        fids.append(abs(np.vdot(zx, zy)) ** 2 / (np.vdot(zx, zx).real * np.vdot(zy, zy).real))
+    plain_cps = len(sub) / t_plain
+    return {
+        "workload": f"{n}-qubit random-layer circuit, 40 layers, seed {SEED}; {count} trajectories, "
+                    f"depth ~ U[10, 40], depolarizing p={p} after every gate per touched qubit "
+                    f"({n_err} Pauli errors sampled); f32 amplitudes, f64 norm and <Z_i Z_i+1>",
+        "circuits_per_s": round(count / r.total_s, 2),
+        "gpu_circuits_per_s": round(count / r.gpu_s, 2),
+        "total_s": round(r.total_s, 3),
+        "plan_s": round(r.plan_s, 3),
+        "checkpoint_s": round(r.checkpoint_s, 3),
+        "gpu_s": round(r.gpu_s, 3),
+        "tasks": r.n_tasks,
+        "shared_clean_trajectories": r.n_shared_clean,
+        "gate_passes": r.n_gate_passes,
+        "dirty_matrices": r.n_dirty_matrices,
+        "slots": r.n_slots,
+        "tuning": {"chunk_layers": best.chunk_layers, "fusion_max": best.fusion_max,
+                   "block": best.block, "slots_per_cu": best.slots_per_cu},
+        "autotune": {"trials": len(trials), "sample": tune_sample, "seconds": round(t_tune, 1)},
+        "kernel_gbs": round(alg / r.gpu_s / 1e9, 1),
+        "kernel_hbm_frac": round(alg / r.gpu_s / 1e9 / HBM_PEAK_GBS, 4),
+        "mean_zz": r.mean_zz,
+        "mean_zz_stderr": float(zz.std(ddof=1) / np.sqrt(len(zz))),
+        "norm_sq_range": [r.min_norm_sq, r.max_norm_sq],
+        "plain_f64_circuits_per_s": round(plain_cps, 2),
+        "plain_sample": len(sub),
+        "speedup_vs_plain_f64": round(count / r.total_s / plain_cps, 2),
+        "fidelity_vs_f64_min": float(min(fids)),
+        "fidelity_sample": len(fids),
+    }
+
+
 def cpu_baseline(gates, n_gpu: int, budget_s: float = 20.0):
     """The oracle's SIMD split-layout kernels (a restatement of the
     reference CPU path) on this host's logical CPUs, over the first fused
@@ -191,6 +277,9 @@ def main() -> int:
     ap.add_argument("--max-size", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f32", action="store_true")
+    ap.add_argument("--no-traj", action="store_true")
+    ap.add_argument("--traj-qubits", type=int, default=24)
+    ap.add_argument("--trajectories", type=int, default=4096)
     ap.add_argument("--copy-probe-qubits", type=int, default=0,
                     help="first copy one f64 statevector of this many qubits device-to-device "
                          "(a known-size transfer that calibrates profiler byte counters)")
@@ -222,6 +311,11 @@ def main() -> int:
     if not args.no_f32:
         r32 = run_gpu(cs, args.qubits, "f32", args.steps, args.warmup, args.max_size)
         log(f"[bench] f32: {r32['gbs_per_gate']:.1f} GB/s per gate, per-k {r32['per_k_gbs']}")
+    traj = None
+    if not args.no_traj:
+        log(f"[bench] trajectories: {args.trajectories} x {args.traj_qubits}q")
+        traj = run_trajectories(cs, args.traj_qubits, args.trajectories,
+                                min(512, args.trajectories), 8)
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
@@ -283,6 +377,8 @@ def main() -> int:
             "per_kernel_gbs": {k: round(v, 1) for k, v in r32["per_k_gbs"].items()},
             "fused_gates": r32["n_gates"],
         }
+    if traj:
+        line["trajectories"] = traj
     print(json.dumps(line), flush=True)
     return 0
 

@@ -270,6 +270,11 @@ __global__ void __launch_bounds__(kMaxBlock)
   if (threadIdx.x < 2 * a.n_pairs)
     s_pairs[threadIdx.x] = a.pairs[threadIdx.x];
   for (;;) {
+    // The leading barrier keeps every path into the lane-0 fetch below
+    // convergent: without it the compiler may thread lanes that skipped a
+    // lane-0-only branch straight to the next barrier, into a loop that waits
+    // for a task index lane 0 never gets to fetch.
+    __syncthreads();
     if (threadIdx.x == 0)
       s_task = atomicAdd(&work[0], 1u);
     __syncthreads();
@@ -292,8 +297,7 @@ __global__ void __launch_bounds__(kMaxBlock)
                 task.n_gates <= a.n_gates - task.first_gate;
     }
     if (!task_ok) {
-      if (threadIdx.x == 0)
-        atomicOr(&work[1], 1u);
+      atomicOr(&work[1], 1u); // every lane: no lane-dependent branch here
       continue;
     }
     const T *cur = checkpoints + 2 * len * task.checkpoint; // state the next gate reads
@@ -313,8 +317,7 @@ __global__ void __launch_bounds__(kMaxBlock)
       }
       ok = ok && uint64_t(moff) + (uint64_t(1) << (2 * k)) <= a.n_matrix_elems;
       if (!ok) {
-        if (threadIdx.x == 0)
-          atomicOr(&work[1], 1u);
+        atomicOr(&work[1], 1u);
         continue;
       }
       const T *mat = matrices + 2 * uint64_t(moff);
