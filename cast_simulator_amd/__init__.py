@@ -47,6 +47,10 @@ from ._host import (  # noqa: E402,F401
     QuantumCircuit,
     QuantumGate,
     Simulator,
+    ServeConfig,
+    ServeRequest,
+    ServeResult,
+    Server,
     SizeOnlyCostModel,
     TrajectoryEngine,
     TrajectoryRun,
@@ -62,6 +66,7 @@ from ._host import (  # noqa: E402,F401
     measure_hip,
     optimize,
     optimize_with_log,
+    poisson_requests,
     random_layer_circuit,
     random_statevector,
     random_unitary_matrix,

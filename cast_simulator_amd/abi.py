@@ -118,6 +118,15 @@ _SIGS = {
     "cast_hip_device_info": (_i, [C.POINTER(DeviceInfo), _err, _sz]),
     "cast_hip_free_memory": (_i, [C.POINTER(_u64), C.POINTER(_u64), _err, _sz]),
     "cast_hip_stream_create": (_vp, [_err, _sz]),
+    "cast_hip_stream_create_nonblocking": (_vp, [_err, _sz]),
+    "cast_hip_event_query": (_i, [_vp, C.POINTER(C.c_int), _err, _sz]),
+    "cast_hip_host_alloc": (_vp, [_sz, _err, _sz]),
+    "cast_hip_host_free": (None, [_vp]),
+    "cast_hip_memcpy_dtoh_async": (_i, [_vp, _u64, _sz, _vp, _err, _sz]),
+    "cast_hip_sv_zero_async": (_i, [_u64, _sz, _u8, _vp, _err, _sz]),
+    "cast_hip_observe_scratch_bytes": (_sz, [_u32]),
+    "cast_hip_observe_async": (
+        _i, [_u64, _u32, _u8, C.POINTER(_u32), _sz, _u64, _u64, _vp, _err, _sz]),
     "cast_hip_stream_destroy": (None, [_vp]),
     "cast_hip_stream_sync": (_i, [_vp, _err, _sz]),
     "cast_hip_event_create": (_vp, [_err, _sz]),
@@ -343,6 +352,32 @@ def memcpy_dtoh(host: np.ndarray, src: int, stream: int | None = None) -> None:
     if not host.flags.c_contiguous:
         raise ValueError("host array must be C-contiguous")
     _call("cast_hip_memcpy_dtoh", host.ctypes.data, src, host.nbytes, stream)
+
+
+def stream_create_nonblocking() -> int:
+    return _ctor("cast_hip_stream_create_nonblocking")
+
+
+def event_query(e: int) -> bool:
+    done = C.c_int(0)
+    _call("cast_hip_event_query", e, C.byref(done))
+    return bool(done.value)
+
+
+def sv_zero_async(dptr: int, n_qubits: int, precision: int, stream: int | None) -> None:
+    _call("cast_hip_sv_zero_async", dptr, 2 << n_qubits, precision, stream)
+
+
+def observe_scratch_bytes(n_pairs: int) -> int:
+    return int(lib().cast_hip_observe_scratch_bytes(n_pairs))
+
+
+def observe_async(dptr: int, n_qubits: int, precision: int, pairs, out: int, scratch: int,
+                  stream: int | None = None) -> None:
+    flat = [q for p in pairs for q in p]
+    arr = (_u32 * max(1, len(flat)))(*flat)
+    _call("cast_hip_observe_async", dptr, n_qubits, precision, arr, len(pairs), out, scratch,
+          stream)
 
 
 def traj_run(args: TrajLaunch, stream: int | None = None) -> None:

@@ -7,6 +7,8 @@
 #include "cast_hip.h"
 #include "common.hpp"
 
+#include <map>
+
 #include <algorithm>
 #include <cmath>
 #include <memory>
@@ -174,7 +176,8 @@ GatePlan *build_plan(const cast_hip_kernel_gen_spec_t *spec, const cast_complex6
   ensure_device();
   check(hipGetDevice(&plan->device), "hipGetDevice");
   const size_t bytes = 2 * K * K * (fp32 ? 4 : 8);
-  check(hipMalloc(&plan->d_mat, bytes), "hipMalloc(gate matrix)");
+  plan->d_mat = cache_alloc(bytes);
+  plan->mat_bytes = bytes;
   if (fp32) {
     std::vector<float> f(2 * K * K);
     for (size_t i = 0; i < f.size(); ++i)
@@ -193,7 +196,8 @@ GatePlan *build_plan(const cast_hip_kernel_gen_spec_t *spec, const cast_complex6
       m3[2 * KK + i] = m[2 * i] + m[2 * i + 1];
     }
     const size_t b3 = 3 * KK * (fp32 ? 4 : 8);
-    check(hipMalloc(&plan->d_mat3, b3), "hipMalloc(gate matrix, 3 real)");
+    plan->d_mat3 = cache_alloc(b3);
+    plan->mat3_bytes = b3;
     if (fp32) {
       std::vector<float> f(m3.begin(), m3.end());
       check(hipMemcpy(plan->d_mat3, f.data(), b3, hipMemcpyHostToDevice), "hipMemcpy(m3)");
@@ -204,13 +208,59 @@ GatePlan *build_plan(const cast_hip_kernel_gen_spec_t *spec, const cast_complex6
   return plan.release();
 }
 
+} // namespace
+
+// Power-of-two size classes from 256 bytes; freed blocks are kept (a plan
+// destroyed while other streams run must not wait for the device).
+struct DevCache {
+  std::mutex mu;
+  std::map<size_t, std::vector<void *>> free_blocks;
+};
+DevCache &dev_cache() {
+  static DevCache *c = new DevCache; // intentionally leaked: outlives static teardown
+  return *c;
+}
+size_t size_class(size_t bytes) {
+  size_t c = 256;
+  while (c < bytes)
+    c <<= 1;
+  return c;
+}
+
+void *cache_alloc(size_t bytes) {
+  const size_t c = size_class(bytes);
+  DevCache &dc = dev_cache();
+  {
+    std::lock_guard<std::mutex> lock(dc.mu);
+    auto &v = dc.free_blocks[c];
+    if (!v.empty()) {
+      void *p = v.back();
+      v.pop_back();
+      return p;
+    }
+  }
+  void *p = nullptr;
+  check(hipMalloc(&p, c), "hipMalloc(gate matrix)");
+  return p;
+}
+
+void cache_release(void *p, size_t bytes) {
+  if (!p)
+    return;
+  DevCache &dc = dev_cache();
+  std::lock_guard<std::mutex> lock(dc.mu);
+  dc.free_blocks[size_class(bytes)].push_back(p);
+}
+
+namespace {
+
 void free_plan(GatePlan *p) {
   if (!p)
     return;
   if (p->d_mat)
-    (void)hipFree(p->d_mat);
+    cache_release(p->d_mat, p->mat_bytes);
   if (p->d_mat3)
-    (void)hipFree(p->d_mat3);
+    cache_release(p->d_mat3, p->mat3_bytes);
   delete[] p->h_mat;
   delete p;
 }
@@ -280,6 +330,31 @@ void cast_hip_stream_destroy(void *stream) {
 int cast_hip_stream_sync(void *stream, char *err, size_t len) {
   return guard(err, len, [&] {
     check(hipStreamSynchronize(static_cast<hipStream_t>(stream)), "hipStreamSynchronize");
+  });
+}
+
+void *cast_hip_stream_create_nonblocking(char *err, size_t len) {
+  hipStream_t s = nullptr;
+  const int rc = guard(err, len, [&] {
+    ensure_device();
+    check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+  });
+  return rc == 0 ? static_cast<void *>(s) : nullptr;
+}
+
+int cast_hip_event_query(void *event, int *out_done, char *err, size_t len) {
+  return guard(err, len, [&] {
+    if (!event || !out_done)
+      throw Error("null event or output");
+    const hipError_t e = hipEventQuery(static_cast<hipEvent_t>(event));
+    if (e == hipSuccess) {
+      *out_done = 1;
+    } else if (e == hipErrorNotReady) {
+      (void)hipGetLastError(); // not an error: clear the sticky status
+      *out_done = 0;
+    } else {
+      check(e, "hipEventQuery");
+    }
   });
 }
 
@@ -374,6 +449,33 @@ int cast_hip_memcpy_dtoh(void *dst, uint64_t src, size_t n_bytes, void *stream, 
   });
 }
 
+void *cast_hip_host_alloc(size_t n_bytes, char *err, size_t len) {
+  void *p = nullptr;
+  const int rc = guard(err, len, [&] {
+    if (n_bytes == 0)
+      throw Error("n_bytes must be positive");
+    ensure_device();
+    check(hipHostMalloc(&p, n_bytes, hipHostMallocDefault), "hipHostMalloc");
+  });
+  return rc == 0 ? p : nullptr;
+}
+
+void cast_hip_host_free(void *ptr) {
+  if (ptr)
+    (void)hipHostFree(ptr);
+}
+
+int cast_hip_memcpy_dtoh_async(void *dst, uint64_t src, size_t n_bytes, void *stream, char *err,
+                               size_t len) {
+  return guard(err, len, [&] {
+    if (!src || (!dst && n_bytes))
+      throw Error("null pointer");
+    check(hipMemcpyAsync(dst, reinterpret_cast<const void *>(src), n_bytes, hipMemcpyDeviceToHost,
+                         static_cast<hipStream_t>(stream)),
+          "hipMemcpyAsync(DtoH)");
+  });
+}
+
 uint64_t cast_hip_sv_alloc(size_t n_elements, uint8_t precision, char *err, size_t len) {
   void *p = nullptr;
   const int rc = guard(err, len, [&] {
@@ -396,6 +498,15 @@ int cast_hip_sv_zero(uint64_t dptr, size_t n_elements, uint8_t precision, char *
     check_sv_args(dptr, n_elements, precision);
     launch_sv_zero(reinterpret_cast<void *>(dptr), n_elements, precision, nullptr);
     check(hipStreamSynchronize(nullptr), "hipStreamSynchronize(zero)");
+  });
+}
+
+int cast_hip_sv_zero_async(uint64_t dptr, size_t n_elements, uint8_t precision, void *stream,
+                           char *err, size_t len) {
+  return guard(err, len, [&] {
+    check_sv_args(dptr, n_elements, precision);
+    launch_sv_zero(reinterpret_cast<void *>(dptr), n_elements, precision,
+                   static_cast<hipStream_t>(stream));
   });
 }
 
@@ -574,6 +685,30 @@ int cast_hip_traj_run(const cast_hip_traj_launch_t *args, void *stream, char *er
     if (!args)
       throw Error("null trajectory launch descriptor");
     launch_traj(*args, static_cast<hipStream_t>(stream));
+  });
+}
+
+size_t cast_hip_observe_scratch_bytes(uint32_t n_pairs) { return observe_scratch_bytes(n_pairs); }
+
+int cast_hip_observe_async(uint64_t sv_dptr, uint32_t sv_n_qubits, uint8_t precision,
+                           const uint32_t *pairs, size_t n_pairs, uint64_t out, uint64_t scratch,
+                           void *stream, char *err, size_t len) {
+  return guard(err, len, [&] {
+    check_precision(precision);
+    if (!sv_dptr || !out || !scratch)
+      throw Error("null statevector, output or scratch pointer");
+    if (n_pairs > 32)
+      throw Error("at most 32 qubit pairs");
+    if (n_pairs > 0 && !pairs)
+      throw Error("null pairs");
+    if (sv_n_qubits < 1 || sv_n_qubits > 40)
+      throw Error("statevector qubit count out of range");
+    for (size_t p = 0; p < 2 * n_pairs; ++p)
+      if (pairs[p] >= sv_n_qubits)
+        throw Error("pair qubit " + std::to_string(pairs[p]) + " outside the statevector");
+    launch_observe(reinterpret_cast<const void *>(sv_dptr), sv_n_qubits, precision, pairs,
+                   n_pairs, reinterpret_cast<double *>(out), reinterpret_cast<double *>(scratch),
+                   static_cast<hipStream_t>(stream));
   });
 }
 

@@ -81,6 +81,7 @@ struct GatePlan {
   int device = 0;
   void *d_mat = nullptr;       // K*K complex, interleaved, device precision
   void *d_mat3 = nullptr;      // [Re M | Im M | Re M + Im M], 3*K*K real (k = 4, 5)
+  size_t mat_bytes = 0, mat3_bytes = 0;
   double *h_mat = nullptr;     // snapped matrix in f64 (host copy, 2*K*K)
   uint64_t scalar_nnz = 0;
   // Monomial form: rows that differ from identity, each with one source col.
@@ -113,6 +114,17 @@ void launch_scale(void *sv, size_t n_elements, uint8_t precision, double scale,
                   hipStream_t stream);
 void expect_zz(const void *sv, uint32_t n_qubits, uint8_t precision, const uint32_t *pairs,
                size_t n_pairs, double *out, hipStream_t stream);
+
+// Deterministic f64 norm + pair sums into device `out` (sv_kernels.hip).
+size_t observe_scratch_bytes(uint32_t n_pairs);
+void launch_observe(const void *sv, uint32_t n_qubits, uint8_t precision, const uint32_t *pairs,
+                    size_t n_pairs, double *out, double *scratch, hipStream_t stream);
+
+// Device blocks for small long-lived objects (gate matrices), cached by
+// power-of-two size class: a released block is kept for reuse rather than
+// returned with hipFree, which waits for the whole device.
+void *cache_alloc(size_t bytes);
+void cache_release(void *p, size_t bytes);
 
 // Persistent batched-trajectory launch (traj_kernels.hip); validates the
 // descriptor and enqueues the counter reset and the kernel.

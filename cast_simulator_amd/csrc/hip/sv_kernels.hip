@@ -135,6 +135,40 @@ __global__ void __launch_bounds__(kRedBlock) k_expect_zz(const T *__restrict__ s
     }
 }
 
+// Norm (slot 0) and pair sums (slots 1..P) per block, written to
+// partial[block * (1 + P) + slot].
+template <typename T>
+__global__ void __launch_bounds__(kRedBlock) k_observe(const T *__restrict__ sv, uint64_t n_amp,
+                                                       PairArgs pa, double *__restrict__ partial) {
+  __shared__ double lds[kRedBlock / 64];
+  using VT = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
+  const VT *amp = reinterpret_cast<const VT *>(sv);
+  double acc[1 + kMaxPairs];
+#pragma unroll
+  for (unsigned p = 0; p < 1 + kMaxPairs; ++p)
+    acc[p] = 0.0;
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < n_amp; j += stride) {
+    const VT v = amp[j];
+    const double re = double(v.x), im = double(v.y);
+    const double pr = __builtin_fma(re, re, im * im);
+    acc[0] += pr;
+#pragma unroll
+    for (unsigned p = 0; p < kMaxPairs; ++p) {
+      const uint64_t par = ((j >> pa.a[p]) ^ (j >> pa.b[p])) & 1u;
+      acc[1 + p] += par ? -pr : pr; // unused slots (a = b = 0) are never read
+    }
+  }
+  const unsigned w = 1 + pa.n_pairs;
+#pragma unroll
+  for (unsigned p = 0; p < 1 + kMaxPairs; ++p)
+    if (p < w) {
+      const double s = block_sum(acc[p], lds);
+      if (threadIdx.x == 0)
+        partial[size_t(blockIdx.x) * w + p] = s;
+    }
+}
+
 unsigned red_grid(uint64_t work) {
   const uint64_t need = (work + kRedBlock - 1) / kRedBlock;
   const uint64_t cap = std::min<uint64_t>(kMaxRedGrid, uint64_t(device_props().n_cu) * 4);
@@ -239,6 +273,34 @@ void expect_zz(const void *sv, uint32_t n_qubits, uint8_t precision, const uint3
     for (uint32_t p = 0; p < pa.n_pairs; ++p)
       out[off + p] = s.h_out[p];
   }
+}
+
+size_t observe_scratch_bytes(uint32_t n_pairs) {
+  return sizeof(double) * kMaxRedGrid * (1 + std::min<uint32_t>(n_pairs, kMaxPairs));
+}
+
+void launch_observe(const void *sv, uint32_t n_qubits, uint8_t precision, const uint32_t *pairs,
+                    size_t n_pairs, double *out, double *scratch, hipStream_t stream) {
+  if (n_pairs > kMaxPairs)
+    throw Error("observe: at most 32 qubit pairs");
+  PairArgs pa{};
+  pa.n_pairs = uint32_t(n_pairs);
+  for (uint32_t p = 0; p < pa.n_pairs; ++p) {
+    pa.a[p] = pairs[2 * p];
+    pa.b[p] = pairs[2 * p + 1];
+  }
+  const uint64_t n_amp = uint64_t(1) << n_qubits;
+  const unsigned grid = red_grid(n_amp);
+  if (precision == 0)
+    hipLaunchKernelGGL(k_observe<float>, dim3(grid), dim3(kRedBlock), 0, stream,
+                       static_cast<const float *>(sv), n_amp, pa, scratch);
+  else
+    hipLaunchKernelGGL(k_observe<double>, dim3(grid), dim3(kRedBlock), 0, stream,
+                       static_cast<const double *>(sv), n_amp, pa, scratch);
+  check(hipGetLastError(), "hipLaunchKernelGGL(k_observe)");
+  hipLaunchKernelGGL(k_sum_partials, dim3(1 + pa.n_pairs), dim3(kRedBlock), 0, stream, scratch,
+                     grid, 1 + pa.n_pairs, out);
+  check(hipGetLastError(), "hipLaunchKernelGGL(k_sum_partials)");
 }
 
 } // namespace cast_hip

@@ -12,6 +12,7 @@
 #include "cost_model.hpp"
 #include "fusion.hpp"
 #include "hip_backend.hpp"
+#include "serving.hpp"
 #include "trajectory.hpp"
 
 #include <cstring>
@@ -577,4 +578,76 @@ PYBIND11_MODULE(_host, m) {
              return py::make_tuple(best, trials);
            },
            py::arg("sample"), py::arg("start"));
+
+  // ── serving ──────────────────────────────────────────────────────────────
+  py::class_<ServeRequest>(m, "ServeRequest")
+      .def(py::init([](uint32_t id, uint32_t n, uint32_t depth, uint64_t seed,
+                       cast_precision_t prec, double arrival_s) {
+             ServeRequest r;
+             r.id = id;
+             r.n_qubits = n;
+             r.depth = depth;
+             r.seed = seed;
+             r.precision = prec;
+             r.arrival_s = arrival_s;
+             return r;
+           }),
+           py::arg("id"), py::arg("n_qubits"), py::arg("depth"), py::arg("seed"),
+           py::arg("precision"), py::arg("arrival_s") = 0.0)
+      .def_readwrite("id", &ServeRequest::id)
+      .def_readwrite("n_qubits", &ServeRequest::n_qubits)
+      .def_readwrite("depth", &ServeRequest::depth)
+      .def_readwrite("seed", &ServeRequest::seed)
+      .def_readwrite("precision", &ServeRequest::precision)
+      .def_readwrite("arrival_s", &ServeRequest::arrival_s);
+  m.def("poisson_requests", &poisson_requests, py::arg("count"), py::arg("rate"),
+        py::arg("n_min"), py::arg("n_max"), py::arg("depth_min"), py::arg("depth_max"),
+        py::arg("f32_fraction"), py::arg("seed"));
+
+  py::class_<ServeConfig>(m, "ServeConfig")
+      .def(py::init<>())
+      .def_readwrite("streams", &ServeConfig::streams)
+      .def_readwrite("compile_threads", &ServeConfig::compile_threads)
+      .def_readwrite("fusion_max", &ServeConfig::fusion_max)
+      .def_readwrite("budget_bytes", &ServeConfig::budget_bytes)
+      .def_readwrite("reserve_bytes", &ServeConfig::reserve_bytes)
+      .def_readwrite("realtime", &ServeConfig::realtime)
+      .def_readwrite("keep_states", &ServeConfig::keep_states);
+
+  py::class_<ServeResult>(m, "ServeResult")
+      .def_readonly("id", &ServeResult::id)
+      .def_readonly("n_qubits", &ServeResult::n_qubits)
+      .def_readonly("depth", &ServeResult::depth)
+      .def_readonly("n_gates", &ServeResult::n_gates)
+      .def_readonly("precision", &ServeResult::precision)
+      .def_readonly("arrival_s", &ServeResult::arrival_s)
+      .def_readonly("compiled_s", &ServeResult::compiled_s)
+      .def_readonly("admit_s", &ServeResult::admit_s)
+      .def_readonly("done_s", &ServeResult::done_s)
+      .def_readonly("norm_sq", &ServeResult::norm_sq)
+      .def_readonly("zz", &ServeResult::zz)
+      .def("state", [](const ServeResult &r) {
+        return py::array_t<double>(py::ssize_t(r.state.size()), r.state.data());
+      });
+
+  py::class_<Server>(m, "Server")
+      .def(py::init([](const ServeConfig &c) { return std::make_unique<Server>(c); }))
+      .def("run", [](Server &s, const std::vector<ServeRequest> &reqs) {
+        ServeStats st;
+        std::vector<ServeResult> res;
+        {
+          py::gil_scoped_release rel;
+          res = s.run(reqs, &st);
+        }
+        py::dict d(py::arg("wall_s") = st.wall_s, py::arg("throughput") = st.throughput,
+                   py::arg("p50_s") = st.p50_s, py::arg("p99_s") = st.p99_s,
+                   py::arg("mean_s") = st.mean_s, py::arg("max_s") = st.max_s,
+                   py::arg("queue_p50_s") = st.queue_p50_s,
+                   py::arg("queue_p99_s") = st.queue_p99_s,
+                   py::arg("max_in_flight") = st.max_in_flight,
+                   py::arg("peak_sv_bytes") = st.peak_sv_bytes,
+                   py::arg("budget_bytes") = st.budget_bytes, py::arg("sv_allocs") = st.sv_allocs,
+                   py::arg("sv_reuses") = st.sv_reuses);
+        return py::make_tuple(res, d);
+      });
 }

@@ -1,0 +1,465 @@
+// Continuous-batching serving loop; see serving.hpp.
+#include "serving.hpp"
+
+#include "fusion.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <thread>
+
+namespace cast::hip {
+
+namespace {
+
+constexpr size_t kErr = 1024;
+constexpr uint32_t kPairs = 32;
+
+double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+uint64_t mix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+size_t sv_bytes_of(uint32_t n, cast_precision_t p) {
+  return (size_t(2) << n) * (p == CAST_PRECISION_F32 ? 4 : 8);
+}
+
+double percentile(std::vector<double> v, double q) {
+  if (v.empty())
+    return 0.0;
+  std::sort(v.begin(), v.end());
+  const size_t k = size_t(std::ceil(q * double(v.size()))) - (q > 0 ? 1 : 0);
+  return v[std::min(k, v.size() - 1)];
+}
+
+} // namespace
+
+std::vector<ServeRequest> poisson_requests(uint32_t count, double rate, uint32_t n_min,
+                                           uint32_t n_max, uint32_t depth_min,
+                                           uint32_t depth_max, double f32_fraction,
+                                           uint64_t seed) {
+  if (!(rate > 0.0) || n_min < 2 || n_min > n_max || n_max > 34 || depth_min < 1 ||
+      depth_min > depth_max || !(f32_fraction >= 0.0 && f32_fraction <= 1.0))
+    throw Error("poisson_requests: invalid parameters");
+  Rng rng(seed);
+  std::vector<ServeRequest> out(count);
+  double t = 0.0;
+  for (uint32_t i = 0; i < count; ++i) {
+    t += rng.exponential(rate);
+    ServeRequest &r = out[i];
+    r.id = i;
+    r.arrival_s = t;
+    r.n_qubits = n_min + uint32_t(rng.below(n_max - n_min + 1));
+    r.depth = depth_min + uint32_t(rng.below(depth_max - depth_min + 1));
+    r.precision = rng.uniform() < f32_fraction ? CAST_PRECISION_F32 : CAST_PRECISION_F64;
+    r.seed = mix64(seed ^ mix64(i + 1));
+  }
+  return out;
+}
+
+// One stream slot: a non-blocking stream with its completion event, the
+// reduction scratch and result buffers of the request it is running.
+struct Server::Slot {
+  void *stream = nullptr, *event = nullptr;
+  uint64_t scratch = 0, out = 0;
+  double *h_out = nullptr;
+  void *h_state = nullptr;
+  size_t h_state_bytes = 0;
+  // current request
+  bool busy = false;
+  size_t req = 0;
+  uint64_t sv = 0;
+  size_t sv_bytes = 0;
+  std::vector<void *> plans;
+};
+
+// Statevector blocks recycled by exact size under a byte budget.
+struct Server::Pool {
+  std::map<size_t, std::vector<uint64_t>> free_blocks;
+  uint64_t budget = 0, in_use = 0, cached = 0, peak = 0, allocs = 0, reuses = 0;
+
+  bool fits(size_t b) const {
+    auto it = free_blocks.find(b);
+    if (it != free_blocks.end() && !it->second.empty())
+      return true;
+    return in_use + b <= budget;
+  }
+  uint64_t acquire(size_t b) {
+    auto it = free_blocks.find(b);
+    if (it != free_blocks.end() && !it->second.empty()) {
+      const uint64_t p = it->second.back();
+      it->second.pop_back();
+      cached -= b;
+      in_use += b;
+      ++reuses;
+      peak = std::max(peak, in_use);
+      return p;
+    }
+    // Trim cached blocks of other sizes until the new one fits the budget.
+    // Freeing waits for the device, so it only happens when the mix of sizes
+    // changes.
+    for (auto &kv : free_blocks) {
+      while (in_use + cached + b > budget && !kv.second.empty()) {
+        cast_hip_free(kv.second.back());
+        kv.second.pop_back();
+        cached -= kv.first;
+      }
+    }
+    if (in_use + cached + b > budget)
+      return 0;
+    char err[kErr];
+    const uint64_t p = cast_hip_malloc(b, err, kErr);
+    if (!p)
+      throw Error(err);
+    in_use += b;
+    ++allocs;
+    peak = std::max(peak, in_use);
+    return p;
+  }
+  void release(uint64_t p, size_t b) {
+    free_blocks[b].push_back(p);
+    in_use -= b;
+    cached += b;
+  }
+  void clear() {
+    for (auto &kv : free_blocks)
+      for (uint64_t p : kv.second)
+        cast_hip_free(p);
+    free_blocks.clear();
+    cached = 0;
+  }
+};
+
+Server::Server(ServeConfig cfg) : cfg_(cfg), pool_(new Pool) {
+  if (cfg_.streams < 1 || cfg_.streams > 64)
+    throw Error("serving: streams must be in 1..64");
+  if (cfg_.compile_threads < 1 || cfg_.compile_threads > 64)
+    throw Error("serving: compile_threads must be in 1..64");
+  if (cfg_.fusion_max < 1 || cfg_.fusion_max > 6)
+    throw Error("serving: fusion_max must be in 1..6");
+  char err[kErr];
+  const size_t scratch = cast_hip_observe_scratch_bytes(kPairs);
+  for (uint32_t i = 0; i < cfg_.streams; ++i) {
+    auto *s = new Slot;
+    slots_.push_back(s);
+    s->stream = cast_hip_stream_create_nonblocking(err, kErr);
+    if (!s->stream)
+      throw Error(err);
+    s->event = cast_hip_event_create(err, kErr);
+    if (!s->event)
+      throw Error(err);
+    s->scratch = cast_hip_malloc(scratch, err, kErr);
+    s->out = s->scratch ? cast_hip_malloc((1 + kPairs) * sizeof(double), err, kErr) : 0;
+    s->h_out = static_cast<double *>(
+        s->out ? cast_hip_host_alloc((1 + kPairs) * sizeof(double), err, kErr) : nullptr);
+    if (!s->h_out)
+      throw Error(err);
+  }
+}
+
+Server::~Server() {
+  char err[kErr];
+  for (Slot *s : slots_) {
+    if (s->stream)
+      (void)cast_hip_stream_sync(s->stream, err, kErr);
+    for (void *p : s->plans)
+      cast_hip_gate_plan_destroy(p);
+    if (s->busy && s->sv)
+      cast_hip_free(s->sv);
+    cast_hip_free(s->scratch);
+    cast_hip_free(s->out);
+    cast_hip_host_free(s->h_out);
+    cast_hip_host_free(s->h_state);
+    if (s->event)
+      cast_hip_event_destroy(s->event);
+    if (s->stream)
+      cast_hip_stream_destroy(s->stream);
+    delete s;
+  }
+  if (pool_)
+    pool_->clear();
+  delete pool_;
+}
+
+std::vector<ServeResult> Server::run(const std::vector<ServeRequest> &reqs, ServeStats *stats) {
+  const size_t N = reqs.size();
+  std::vector<ServeResult> res(N);
+  if (N == 0)
+    return res;
+  // Requests in arrival order.
+  std::vector<size_t> order(N);
+  for (size_t i = 0; i < N; ++i)
+    order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t a, size_t b) { return reqs[a].arrival_s < reqs[b].arrival_s; });
+
+  char err[kErr];
+  uint64_t free_b = 0, total_b = 0;
+  throw_if(cast_hip_free_memory(&free_b, &total_b, err, kErr), err);
+  Pool &pool = *pool_;
+  const uint64_t avail = free_b + pool.cached > cfg_.reserve_bytes
+                             ? free_b + pool.cached - cfg_.reserve_bytes
+                             : 0;
+  pool.budget = cfg_.budget_bytes ? std::min<uint64_t>(cfg_.budget_bytes, avail + pool.in_use)
+                                  : avail + pool.in_use;
+  pool.peak = pool.in_use;
+  for (const ServeRequest &r : reqs) {
+    if (r.n_qubits < 2 || r.n_qubits > 34 || r.depth < 1)
+      throw Error("serving: request " + std::to_string(r.id) + " has an invalid shape");
+    if (sv_bytes_of(r.n_qubits, r.precision) > pool.budget)
+      throw Error("serving: request " + std::to_string(r.id) + " needs more memory than the "
+                  "statevector budget");
+  }
+
+  // ── compile workers ──────────────────────────────────────────────────────
+  struct Compiled {
+    std::vector<void *> plans;
+    std::string error;
+  };
+  std::vector<Compiled> compiled(N);
+  std::mutex mu;
+  std::condition_variable cv;
+  std::map<size_t, size_t> ready; // arrival rank -> request index
+  std::atomic<size_t> next{0};
+  std::atomic<bool> abort{false};
+  const double t0 = now_s();
+  auto worker = [&] {
+    for (;;) {
+      const size_t rank = next.fetch_add(1);
+      if (rank >= N || abort.load())
+        return;
+      const size_t i = order[rank];
+      const ServeRequest &r = reqs[i];
+      if (cfg_.realtime) {
+        const double wait = t0 + r.arrival_s - now_s();
+        if (wait > 0)
+          std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+      }
+      Compiled c;
+      try {
+        CircuitGraph g = CircuitGraph::from_circuit(random_layer_circuit(r.n_qubits, r.depth, r.seed));
+        optimize(g, FusionConfig::size_only(cfg_.fusion_max));
+        const HipKernelGenSpec spec =
+            r.precision == CAST_PRECISION_F32 ? HipKernelGenSpec::f32() : HipKernelGenSpec::f64();
+        cast_hip_kernel_gen_spec_t cs{};
+        cs.precision = spec.precision;
+        cs.ztol = spec.ztol;
+        cs.otol = spec.otol;
+        cs.variant = CAST_HIP_VARIANT_AUTO;
+        char e[kErr];
+        for (const GatePtr &gp : g.gates_in_row_order()) {
+          const auto &m = gp->matrix().data();
+          std::vector<cast_complex64_t> cm(m.size());
+          for (size_t k = 0; k < m.size(); ++k)
+            cm[k] = {m[k].real(), m[k].imag()};
+          void *plan = nullptr;
+          if (cast_hip_gate_plan_create(&cs, cm.data(), cm.size(), gp->qubits().data(),
+                                        gp->qubits().size(), &plan, e, kErr) != 0)
+            throw Error(e);
+          c.plans.push_back(plan);
+        }
+      } catch (const std::exception &ex) {
+        c.error = ex.what();
+      }
+      const double tc = now_s() - t0;
+      {
+        std::lock_guard<std::mutex> lock(mu);
+        compiled[i] = std::move(c);
+        res[i].compiled_s = tc;
+        ready[rank] = i;
+      }
+      cv.notify_one();
+    }
+  };
+  std::vector<std::thread> threads;
+  for (uint32_t k = 0; k < cfg_.compile_threads; ++k)
+    threads.emplace_back(worker);
+
+  // ── scheduler ───────────────────────────────────────────────────────────
+  size_t done = 0;
+  uint32_t in_flight = 0, max_in_flight = 0;
+  std::string failure;
+  auto finish = [&](Slot &s) {
+    ServeResult &o = res[s.req];
+    const ServeRequest &r = reqs[s.req];
+    o.done_s = now_s() - t0;
+    o.norm_sq = s.h_out[0];
+    o.zz.assign(s.h_out + 1, s.h_out + r.n_qubits);
+    if (cfg_.keep_states) {
+      const size_t n_el = size_t(2) << r.n_qubits;
+      o.state.resize(n_el);
+      if (r.precision == CAST_PRECISION_F32) {
+        const float *f = static_cast<const float *>(s.h_state);
+        std::copy(f, f + n_el, o.state.begin());
+      } else {
+        std::memcpy(o.state.data(), s.h_state, n_el * sizeof(double));
+      }
+    }
+    for (void *p : s.plans)
+      cast_hip_gate_plan_destroy(p);
+    s.plans.clear();
+    pool.release(s.sv, s.sv_bytes);
+    s.sv = 0;
+    s.busy = false;
+    --in_flight;
+    ++done;
+  };
+  auto admit = [&](Slot &s, size_t i, std::vector<void *> plans) {
+    const ServeRequest &r = reqs[i];
+    const size_t b = sv_bytes_of(r.n_qubits, r.precision);
+    s.sv = pool.acquire(b);
+    if (!s.sv)
+      throw Error("serving: statevector pool refused an admitted request");
+    s.sv_bytes = b;
+    s.req = i;
+    s.plans = std::move(plans);
+    s.busy = true;
+    ++in_flight;
+    max_in_flight = std::max(max_in_flight, in_flight);
+    ServeResult &o = res[i];
+    o.id = r.id;
+    o.n_qubits = r.n_qubits;
+    o.depth = r.depth;
+    o.precision = r.precision;
+    o.arrival_s = r.arrival_s;
+    o.n_gates = uint32_t(s.plans.size());
+    o.admit_s = now_s() - t0;
+    const uint8_t prec = uint8_t(r.precision);
+    throw_if(cast_hip_sv_zero_async(s.sv, size_t(2) << r.n_qubits, prec, s.stream, err, kErr),
+             err);
+    for (void *p : s.plans)
+      throw_if(cast_hip_gate_apply(p, s.stream, s.sv, r.n_qubits, err, kErr), err);
+    uint32_t pairs[2 * kPairs];
+    const uint32_t np = r.n_qubits - 1;
+    for (uint32_t j = 0; j < np; ++j) {
+      pairs[2 * j] = j;
+      pairs[2 * j + 1] = j + 1;
+    }
+    throw_if(cast_hip_observe_async(s.sv, r.n_qubits, prec, pairs, np, s.out, s.scratch, s.stream,
+                                    err, kErr),
+             err);
+    throw_if(cast_hip_memcpy_dtoh_async(s.h_out, s.out, (1 + np) * sizeof(double), s.stream, err,
+                                        kErr),
+             err);
+    if (cfg_.keep_states) {
+      if (s.h_state_bytes < b) {
+        cast_hip_host_free(s.h_state);
+        s.h_state = cast_hip_host_alloc(b, err, kErr);
+        if (!s.h_state)
+          throw Error(err);
+        s.h_state_bytes = b;
+      }
+      throw_if(cast_hip_memcpy_dtoh_async(s.h_state, s.sv, b, s.stream, err, kErr), err);
+    }
+    throw_if(cast_hip_event_record(s.event, s.stream, err, kErr), err);
+  };
+
+  try {
+    while (done < N) {
+      bool progress = false;
+      for (Slot *s : slots_) {
+        if (!s->busy)
+          continue;
+        int fin = 0;
+        throw_if(cast_hip_event_query(s->event, &fin, err, kErr), err);
+        if (fin) {
+          finish(*s);
+          progress = true;
+        }
+      }
+      for (Slot *s : slots_) {
+        if (s->busy)
+          continue;
+        // Oldest compiled request that fits; younger ones may pass an older
+        // one only when the older one's statevector does not fit yet.
+        size_t pick = SIZE_MAX;
+        std::vector<void *> plans;
+        {
+          std::lock_guard<std::mutex> lock(mu);
+          for (auto it = ready.begin(); it != ready.end(); ++it) {
+            const size_t i = it->second;
+            if (!compiled[i].error.empty())
+              throw Error("serving: compiling request " + std::to_string(reqs[i].id) + ": " +
+                          compiled[i].error);
+            if (pool.fits(sv_bytes_of(reqs[i].n_qubits, reqs[i].precision))) {
+              pick = i;
+              plans = std::move(compiled[i].plans);
+              ready.erase(it);
+              break;
+            }
+          }
+        }
+        if (pick == SIZE_MAX)
+          break;
+        admit(*s, pick, std::move(plans));
+        progress = true;
+      }
+      if (!progress) {
+        std::unique_lock<std::mutex> lock(mu);
+        cv.wait_for(lock, std::chrono::microseconds(in_flight ? 20 : 500));
+      }
+    }
+  } catch (const std::exception &ex) {
+    failure = ex.what();
+  }
+  abort = true;
+  for (auto &t : threads)
+    t.join();
+  if (!failure.empty()) {
+    // Drain the device, then release everything this run still holds.
+    for (Slot *s : slots_) {
+      (void)cast_hip_stream_sync(s->stream, err, kErr);
+      if (s->busy) {
+        for (void *p : s->plans)
+          cast_hip_gate_plan_destroy(p);
+        s->plans.clear();
+        pool.release(s->sv, s->sv_bytes);
+        s->busy = false;
+      }
+    }
+    for (auto &c : compiled)
+      for (void *p : c.plans)
+        cast_hip_gate_plan_destroy(p);
+    throw Error(failure);
+  }
+  const double wall = now_s() - t0;
+  if (stats) {
+    std::vector<double> lat, queue;
+    for (const ServeResult &o : res) {
+      lat.push_back(o.done_s - o.arrival_s);
+      queue.push_back(o.admit_s - o.arrival_s);
+    }
+    ServeStats &st = *stats;
+    st.wall_s = wall;
+    st.throughput = double(N) / wall;
+    st.p50_s = percentile(lat, 0.50);
+    st.p99_s = percentile(lat, 0.99);
+    st.max_s = *std::max_element(lat.begin(), lat.end());
+    double m = 0.0;
+    for (double x : lat)
+      m += x;
+    st.mean_s = m / double(N);
+    st.queue_p50_s = percentile(queue, 0.50);
+    st.queue_p99_s = percentile(queue, 0.99);
+    st.max_in_flight = max_in_flight;
+    st.peak_sv_bytes = pool.peak;
+    st.budget_bytes = pool.budget;
+    st.sv_allocs = pool.allocs;
+    st.sv_reuses = pool.reuses;
+  }
+  return res;
+}
+
+} // namespace cast::hip

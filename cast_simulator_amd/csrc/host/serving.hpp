@@ -1,0 +1,93 @@
+// Continuous-batching serving loop for circuit simulation on one GPU.
+//
+// The reference runs one circuit at a time (src/simulator/mod.rs:523-547:
+// compile, allocate, apply every kernel, synchronise).  Here requests arrive
+// over time (seeded Poisson process), are compiled on a pool of host threads
+// as they arrive (circuit -> fusion -> gate plans), and are admitted to the
+// GPU while earlier requests are still running, as soon as a stream slot is
+// free and their statevector fits the device-memory budget.  Each admitted
+// request owns one non-blocking stream and a resident statevector for its
+// lifetime: |0> fill, every fused gate, and a deterministic f64 reduction of
+// the norm and <Z_i Z_i+1> are queued back to back, followed by an event
+// that the scheduler polls.  Statevector blocks are recycled by size class,
+// so steady-state serving never calls the device allocator or its free
+// (which would wait for the whole device).
+//
+// Every request's result depends only on its own circuit: the kernels are
+// deterministic, so it is the same alone and in any mix.
+#pragma once
+
+#include "hip_backend.hpp"
+
+#include <cstdint>
+#include <vector>
+
+namespace cast::hip {
+
+struct ServeRequest {
+  uint32_t id = 0;
+  uint32_t n_qubits = 0;
+  uint32_t depth = 0;
+  uint64_t seed = 0; // random_layer_circuit(n_qubits, depth, seed)
+  cast_precision_t precision = CAST_PRECISION_F64;
+  double arrival_s = 0.0; // offset from the start of the run
+};
+
+// `count` requests with exponential inter-arrival gaps of mean 1/rate,
+// n ~ U{n_min..n_max}, depth ~ U{depth_min..depth_max}, f32 with
+// probability f32_fraction, all drawn from one generator seeded by `seed`.
+std::vector<ServeRequest> poisson_requests(uint32_t count, double rate, uint32_t n_min,
+                                           uint32_t n_max, uint32_t depth_min,
+                                           uint32_t depth_max, double f32_fraction,
+                                           uint64_t seed);
+
+struct ServeConfig {
+  uint32_t streams = 8;          // concurrent requests on the device
+  uint32_t compile_threads = 8;  // host threads compiling arrived requests
+  uint32_t fusion_max = 5;       // size-only fusion bound (1..6)
+  uint64_t budget_bytes = 0;     // statevector memory; 0 = free memory - reserve
+  uint64_t reserve_bytes = uint64_t(8) << 30;
+  bool realtime = true;          // release requests at their arrival times
+  bool keep_states = false;      // copy every final statevector to the host
+};
+
+struct ServeResult {
+  uint32_t id = 0, n_qubits = 0, depth = 0, n_gates = 0;
+  cast_precision_t precision = CAST_PRECISION_F64;
+  double arrival_s = 0.0, compiled_s = 0.0, admit_s = 0.0, done_s = 0.0;
+  double norm_sq = 0.0;
+  std::vector<double> zz;    // <Z_i Z_i+1> sums, i = 0 .. n-2 (unnormalised)
+  std::vector<double> state; // interleaved f64, when keep_states
+};
+
+struct ServeStats {
+  double wall_s = 0.0;
+  double throughput = 0.0;       // completed requests / wall
+  double p50_s = 0.0, p99_s = 0.0, mean_s = 0.0, max_s = 0.0; // arrival -> result
+  double queue_p50_s = 0.0, queue_p99_s = 0.0;                 // arrival -> admission
+  uint32_t max_in_flight = 0;
+  uint64_t peak_sv_bytes = 0;
+  uint64_t budget_bytes = 0;
+  uint64_t sv_allocs = 0, sv_reuses = 0;
+};
+
+class Server {
+public:
+  explicit Server(ServeConfig cfg);
+  ~Server();
+  Server(const Server &) = delete;
+  Server &operator=(const Server &) = delete;
+
+  // Serves every request (in arrival order); returns results by request id
+  // order of `requests` and the latency statistics.
+  std::vector<ServeResult> run(const std::vector<ServeRequest> &requests, ServeStats *stats);
+
+private:
+  struct Slot;
+  struct Pool;
+  ServeConfig cfg_;
+  std::vector<Slot *> slots_;
+  Pool *pool_ = nullptr;
+};
+
+} // namespace cast::hip

@@ -126,7 +126,14 @@ int cast_hip_stream_sync(void *stream, char *err_buf, size_t err_buf_len);
 
 /* ── Events (ffi_cuda.h:66-71) ───────────────────────────────────────────── */
 
+/* A stream that does not synchronise with the null stream, so work queued on
+ * it overlaps with synchronous null-stream calls made by other host threads
+ * (plan uploads while requests are in flight).  No reference counterpart. */
+void *cast_hip_stream_create_nonblocking(char *err_buf, size_t err_buf_len);
 void *cast_hip_event_create(char *err_buf, size_t err_buf_len);
+/* *out_done = 1 when all work captured by the event has completed, else 0;
+ * never blocks.  No reference counterpart (the reference only synchronises). */
+int cast_hip_event_query(void *event, int *out_done, char *err_buf, size_t err_buf_len);
 int cast_hip_event_record(void *event, void *stream, char *err_buf, size_t err_buf_len);
 int cast_hip_event_synchronize(void *event, char *err_buf, size_t err_buf_len);
 void cast_hip_event_destroy(void *event);
@@ -151,6 +158,13 @@ int cast_hip_memcpy_htod_async(uint64_t dst, const void *src, size_t n_bytes, vo
 /* Device -> host copy; returns after completion. */
 int cast_hip_memcpy_dtoh(void *dst, uint64_t src, size_t n_bytes, void *stream, char *err_buf,
                          size_t err_buf_len);
+/* Page-locked host memory, for copies that overlap with device work. */
+void *cast_hip_host_alloc(size_t n_bytes, char *err_buf, size_t err_buf_len); /* NULL on failure */
+void cast_hip_host_free(void *ptr);
+/* Device -> host copy queued on `stream`; dst must be page-locked
+ * (cast_hip_host_alloc) and stays undefined until the stream reaches it. */
+int cast_hip_memcpy_dtoh_async(void *dst, uint64_t src, size_t n_bytes, void *stream,
+                               char *err_buf, size_t err_buf_len);
 
 /* ── Statevector memory (ffi_cuda.h:80-88) ───────────────────────────────── */
 
@@ -162,6 +176,9 @@ void cast_hip_sv_free(uint64_t dptr);
  * Runs on the null stream and returns after completion. */
 int cast_hip_sv_zero(uint64_t dptr, size_t n_elements, uint8_t precision, char *err_buf,
                      size_t err_buf_len);
+/* |0...0> queued on `stream` (non-blocking). */
+int cast_hip_sv_zero_async(uint64_t dptr, size_t n_elements, uint8_t precision, void *stream,
+                           char *err_buf, size_t err_buf_len);
 int cast_hip_sv_upload(uint64_t dptr, const double *host_data, size_t n_elements,
                        uint8_t precision, char *err_buf, size_t err_buf_len);
 int cast_hip_sv_download(uint64_t dptr, double *host_data, size_t n_elements, uint8_t precision,
@@ -220,6 +237,17 @@ int cast_hip_gate_apply_batch(const void *const *plans, const uint64_t *svs,
 int cast_hip_expect_zz(uint64_t sv_dptr, uint32_t sv_n_qubits, uint8_t precision,
                        const uint32_t *pairs, size_t n_pairs, double *out, void *stream,
                        char *err_buf, size_t err_buf_len);
+
+/* Queued on `stream`: out[0] = sum |amp|^2 and out[1 + j] = sum |amp|^2
+ * (-1)^(bit a_j xor bit b_j), f64, into device memory `out` (1 + n_pairs
+ * doubles, n_pairs <= 32), using device `scratch` of
+ * cast_hip_observe_scratch_bytes(n_pairs) bytes that no other queued work
+ * uses until this completes.  Deterministic (fixed reduction order).  No
+ * reference counterpart. */
+size_t cast_hip_observe_scratch_bytes(uint32_t n_pairs);
+int cast_hip_observe_async(uint64_t sv_dptr, uint32_t sv_n_qubits, uint8_t precision,
+                           const uint32_t *pairs, size_t n_pairs, uint64_t out, uint64_t scratch,
+                           void *stream, char *err_buf, size_t err_buf_len);
 
 /* ── Batched trajectories in one persistent launch ───────────────────────────
  * No reference counterpart (the reference branches deterministic ensembles
