@@ -225,6 +225,55 @@ def run_trajectories(cs, n: int, count: int, tune_sample: int, plain_sample: int
     }
 
 
+def run_serving(cs, count: int, rate: float, capacity_sample: int):
+    """Continuous-batching serving of random-layer circuits.
+
+    Workload: seeded Poisson arrivals at `rate` req/s; n ~ U{12..28}, depth ~
+    U{10..50}, f32 or f64 with probability 1/2 each; statevectors resident
+    on the device from admission to result.  Latency = arrival -> result
+    (f64 norm and <Z_i Z_i+1> on the host).  Capacity = the first
+    `capacity_sample` requests all queued at t = 0, served with overlap,
+    against the plain per-circuit path: the same circuits in f64, one at a
+    time on one stream."""
+    reqs = cs.poisson_requests(count, rate, 12, 28, 10, 50, 0.5, SEED)
+
+    def serve(rs, streams, threads, realtime):
+        cfg = cs.ServeConfig()
+        cfg.streams, cfg.compile_threads, cfg.realtime = streams, threads, realtime
+        return cs.Server(cfg).run(rs)
+
+    res, st = serve(reqs, 8, 16, True)
+    log(f"  serving: {count} requests at {rate}/s: {st['throughput']:.1f}/s, "
+        f"p50 {st['p50_s'] * 1e3:.1f} ms, p99 {st['p99_s'] * 1e3:.1f} ms")
+    sub = reqs[:capacity_sample]
+    _, cap = serve(sub, 8, 16, False)
+    plain = []
+    for r in sub:
+        plain.append(cs.ServeRequest(r.id, r.n_qubits, r.depth, r.seed, cs.Precision.F64, 0.0))
+    _, pl = serve(plain, 1, 1, False)
+    log(f"  capacity: {cap['throughput']:.1f} circuits/s mixed, plain f64 serial "
+        f"{pl['throughput']:.1f} circuits/s")
+    norms = [abs(o.norm_sq - 1.0) for o in res]
+    return {
+        "workload": f"{count} requests, Poisson arrivals at {rate}/s (seed {SEED}), "
+                    "random-layer circuits n ~ U{12..28}, depth ~ U{10..50}, f32/f64 50/50, "
+                    "fused to k<=5, resident statevectors, admission by free HBM",
+        "circuits_per_s": round(st["throughput"], 2),
+        "p50_ms": round(st["p50_s"] * 1e3, 2),
+        "p99_ms": round(st["p99_s"] * 1e3, 2),
+        "mean_ms": round(st["mean_s"] * 1e3, 2),
+        "queue_p99_ms": round(st["queue_p99_s"] * 1e3, 2),
+        "max_in_flight": st["max_in_flight"],
+        "peak_sv_gib": round(st["peak_sv_bytes"] / 2**30, 2),
+        "wall_s": round(st["wall_s"], 2),
+        "capacity_circuits_per_s": round(cap["throughput"], 2),
+        "capacity_sample": len(sub),
+        "plain_f64_serial_circuits_per_s": round(pl["throughput"], 2),
+        "speedup_vs_plain_f64": round(cap["throughput"] / pl["throughput"], 2),
+        "max_norm_error": max(norms),
+    }
+
+
 def cpu_baseline(gates, n_gpu: int, budget_s: float = 20.0):
     """The oracle's SIMD split-layout kernels (a restatement of the
     reference CPU path) on this host's logical CPUs, over the first fused
@@ -278,6 +327,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f32", action="store_true")
     ap.add_argument("--no-traj", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--requests", type=int, default=1000)
+    ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--traj-qubits", type=int, default=24)
     ap.add_argument("--trajectories", type=int, default=4096)
     ap.add_argument("--copy-probe-qubits", type=int, default=0,
@@ -316,6 +368,10 @@ def main() -> int:
         log(f"[bench] trajectories: {args.trajectories} x {args.traj_qubits}q")
         traj = run_trajectories(cs, args.traj_qubits, args.trajectories,
                                 min(512, args.trajectories), 8)
+    serve = None
+    if not args.no_serve:
+        log(f"[bench] serving: {args.requests} requests at {args.rate}/s")
+        serve = run_serving(cs, args.requests, args.rate, min(200, args.requests))
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
@@ -379,6 +435,8 @@ def main() -> int:
         }
     if traj:
         line["trajectories"] = traj
+    if serve:
+        line["serving"] = serve
     print(json.dumps(line), flush=True)
     return 0
 

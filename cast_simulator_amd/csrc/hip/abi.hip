@@ -162,8 +162,8 @@ GatePlan *build_plan(const cast_hip_kernel_gen_spec_t *spec, const cast_complex6
     throw Error("variant MONOMIAL requested for a matrix with >1 nonzero in a row");
   if (variant == CAST_HIP_VARIANT_TILE && n_qubits > 5)
     variant = CAST_HIP_VARIANT_DENSE; // wave tiles exist for k <= 5
-  if (variant == CAST_HIP_VARIANT_MFMA && (n_qubits < 4 || n_qubits > 5))
-    variant = CAST_HIP_VARIANT_DENSE; // matrix-core tiles exist for k = 4, 5
+  if (variant == CAST_HIP_VARIANT_MFMA && n_qubits < 4)
+    variant = CAST_HIP_VARIANT_DENSE; // matrix-core tiles exist for k = 4, 5, 6
   plan->variant = variant;
   plan->forced = spec->variant != CAST_HIP_VARIANT_AUTO;
   if (spec->flags & ~CAST_HIP_FLAG_TILE_SWIZZLE)
@@ -186,7 +186,7 @@ GatePlan *build_plan(const cast_hip_kernel_gen_spec_t *spec, const cast_complex6
   } else {
     check(hipMemcpy(plan->d_mat, m.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy(matrix)");
   }
-  if (variant != CAST_HIP_VARIANT_MONOMIAL && (n_qubits == 4 || n_qubits == 5)) {
+  if (variant != CAST_HIP_VARIANT_MONOMIAL && n_qubits >= 4) {
     // Real matrices of the three-product form: Re M, Im M, Re M + Im M.
     const size_t KK = K * K;
     std::vector<double> m3(3 * KK);

@@ -226,6 +226,73 @@ __global__ void __launch_bounds__(256) k_mfma(T *__restrict__ sv, const T *__res
   }
 }
 
+// ── Matrix cores, k = 6: real matrices in LDS ──────────────────────────────
+//
+// Same three-product contraction as k_mfma for K = 64, where the A operands
+// (3 x 4 x 16 values per lane) no longer fit in registers.  Re M and Im M are
+// staged once per workgroup in LDS, laid out so that operand (m, rb, s) of
+// lane l is element ((m * 4 + rb) * 16 + s) * 64 + l: every read is 64
+// consecutive values (no bank conflicts).  Re M + Im M is formed on the fly,
+// which keeps the f64 staging area at 64 KiB.  The 16 input amplitudes of a
+// lane stay in registers while the four 16-row output blocks are produced.
+
+constexpr unsigned kMfma6Block = 512;
+
+template <typename T>
+__global__ void __launch_bounds__(kMfma6Block) k_mfma6(T *__restrict__ sv, const T *__restrict__ m3,
+                                                       GateGeom g, uint64_t n_tiles) {
+  constexpr int KL = 6, K = 64, RB = 4, KS = 16;
+  using VT = typename V2<T>::type;
+  using AV = typename Acc4<T>::type;
+  __shared__ T a_lds[2 * RB * KS * 64];
+  for (unsigned i = threadIdx.x; i < 2 * RB * KS * 64; i += blockDim.x) {
+    const unsigned l = i & 63u, s = (i >> 6) & (KS - 1), rb = (i >> 10) & (RB - 1), m = i >> 12;
+    a_lds[i] = m3[m * K * K + (16 * rb + (l & 15u)) * K + 4 * s + (l >> 4)];
+  }
+  __syncthreads();
+  VT *__restrict__ amp = reinterpret_cast<VT *>(sv);
+  const unsigned lane = threadIdx.x & 63u, j = lane >> 4, col = lane & 15u;
+  uint64_t din[KS];
+#pragma unroll
+  for (int s = 0; s < KS; ++s)
+    din[s] = local_delta<KL>(4 * s + j, g.q);
+  const uint64_t n_waves = (uint64_t(gridDim.x) * blockDim.x) >> 6;
+  for (uint64_t t = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6; t < n_tiles;
+       t += n_waves) {
+    const uint64_t base = combo_base<KL>(t * 16 + col, g.q);
+    T br[KS], bi[KS], bs[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const VT x = amp[base + din[s]];
+      br[s] = x.x;
+      bi[s] = x.y;
+      bs[s] = x.x + x.y;
+    }
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb) {
+      // Re-read per tile (not hoisted out of the tile loop into 128+ VGPRs).
+      const T *a = a_lds + rb * KS * 64 + lane + dev::opaque_u32(0);
+      AV t1 = {0, 0, 0, 0}, t2 = {0, 0, 0, 0}, t3 = {0, 0, 0, 0};
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        const T ar = a[s * 64];
+        const T ai = a[RB * KS * 64 + s * 64];
+        t1 = mfma16(ar, br[s], t1);
+        t2 = mfma16(ai, bi[s], t2);
+        t3 = mfma16(ar + ai, bs[s], t3);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        VT o;
+        o.x = t1[i] - t2[i];
+        o.y = t3[i] - t1[i] - t2[i];
+        const uint32_t row = 16 * rb + (sizeof(T) == 8 ? j + 4 * i : 4 * j + i);
+        amp[base + local_delta<KL>(row, g.q)] = o;
+      }
+    }
+  }
+}
+
 // ── Batched dense: item = blockIdx.y ────────────────────────────────────────
 
 constexpr int kMaxBatch = 64;
@@ -319,6 +386,17 @@ void launch_mfma_t(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &g
   check(hipGetLastError(), "hipLaunchKernelGGL(k_mfma)");
 }
 
+template <typename T>
+void launch_mfma6_t(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &g) {
+  const uint64_t tiles = g.combos / 16;
+  const uint64_t need = (tiles + kMfma6Block / 64 - 1) / (kMfma6Block / 64);
+  const uint64_t cap = uint64_t(std::max(1, device_props().n_cu)) * 2; // LDS: 2 blocks per CU
+  const unsigned grid = unsigned(std::max<uint64_t>(1, std::min(need, cap)));
+  hipLaunchKernelGGL((k_mfma6<T>), dim3(grid), dim3(kMfma6Block), 0, s, static_cast<T *>(sv),
+                     static_cast<const T *>(p.d_mat3), g, tiles);
+  check(hipGetLastError(), "hipLaunchKernelGGL(k_mfma6)");
+}
+
 template <typename T, int KL, int L>
 void launch_tile_kl(const GatePlan &p, hipStream_t s, void *sv, const TileGeom &tg) {
   const uint64_t blocks = (tg.n_tiles + 3) / 4; // 4 waves per block, one tile per wave
@@ -397,7 +475,7 @@ void launch_tile(const GatePlan &p, hipStream_t s, void *sv, uint32_t n) {
 
 // Kernel family for a dense plan on an n-qubit statevector.  A forced family
 // falls back to the direct kernel where its shape requirement is not met
-// (wave tiles need n >= k + 6, matrix-core tiles k in {4, 5} and n >= k + 4).
+// (wave tiles need n >= k + 6, matrix-core tiles k in {4, 5, 6} and n >= k + 4).
 uint32_t dense_family(const GatePlan &p, uint32_t n) {
   const bool tile_ok = p.k <= 5 && n >= p.k + 6;
   const bool mfma_ok = p.d_mat3 != nullptr && n >= p.k + 4;
@@ -412,7 +490,9 @@ uint32_t dense_family(const GatePlan &p, uint32_t n) {
     //  * wave tiles stream full 1 KiB rows and win for up to 3 low-order
     //    targets; with 4 or 5 the exchanged register groups cost more than
     //    the matrix-core kernel's 256-byte segments.
-    if (mfma_ok && (p.precision == 0 || n_low >= 4))
+    //  * k = 6: only the matrix-core kernel keeps its operands out of
+    //    scratch memory.
+    if (mfma_ok && (p.precision == 0 || n_low >= 4 || p.k == 6))
       v = 4;
     else if (tile_ok && n_low <= 3)
       v = 3;
@@ -435,7 +515,9 @@ void launch_by_k(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &g) 
     if (fam == 4) {
       if (p.k == 4)
         return launch_mfma_t<T, 4>(p, s, sv, g);
-      return launch_mfma_t<T, 5>(p, s, sv, g);
+      if (p.k == 5)
+        return launch_mfma_t<T, 5>(p, s, sv, g);
+      return launch_mfma6_t<T>(p, s, sv, g);
     }
     if (fam == 3)
       return launch_tile<T>(p, s, sv, n);

@@ -83,7 +83,7 @@ typedef struct cast_hip_kernel_gen_spec_t {
 #define CAST_HIP_VARIANT_MONOMIAL 2u /* <=1 nonzero per row (diag / permutation with phases)    */
 #define CAST_HIP_VARIANT_TILE 3u     /* dense, coalesced 64*K wave tiles; low-order targets      */
                                      /* exchanged across lanes (k <= 5, n >= k + 6)            */
-#define CAST_HIP_VARIANT_MFMA 4u     /* dense k = 4, 5 on matrix cores, three real products     */
+#define CAST_HIP_VARIANT_MFMA 4u     /* dense k = 4..6 on matrix cores, three real products     */
 
 /* Read-only description of a built plan (no reference counterpart: the
  * reference exposes this through PTX text, src/cuda/kernel.rs:847-882). */

@@ -466,3 +466,25 @@ def test_roundtrip_at_30_qubits(f32):
     assert abs(sv.norm_squared() - 1.0) < tol
     zz = np.asarray(sv.expect_zz([0, 1, 4, 5, 28, 29, 0, 29]))
     assert np.all(np.abs(zz - 1.0) < tol)
+
+
+@pytest.mark.parametrize("f32", [False, True], ids=["f64", "f32"])
+@pytest.mark.parametrize("qubits", [[0, 1, 2, 3, 4, 5], [2, 9, 14, 17, 20, 23], [18, 19, 20, 21, 22, 23]],
+                         ids=["low", "mixed", "high"])
+def test_mfma_k6_at_24_qubits_matches_direct_kernel(qubits, f32):
+    """k = 6 on matrix cores with several 16-combo tiles per wave (2^24
+    amplitudes) against the direct kernel (itself pinned to the oracle on the
+    same shapes above)."""
+    n = 24
+    g = cs.QuantumGate.random_unitary(qubits, 77 + sum(qubits))
+    init = np.asarray(cs.random_statevector(n, 5))
+    print(f"[parity] k=6 {qubits} n={n} f32={f32}", flush=True)
+    a = apply_abi(g.matrix(), qubits, init, f32, abi.VARIANT_MFMA, n)
+    b = apply_abi(g.matrix(), qubits, init, f32, abi.VARIANT_DENSE, n)
+    assert_amps_close(b, a, TOL32 if f32 else TOL64, f"k6 {qubits}")
+    plan = abi.plan_create(np.asarray(g.matrix()), qubits, abi.F32 if f32 else abi.F64,
+                           variant=abi.VARIANT_AUTO)
+    try:
+        assert abi.plan_family(plan, n) == abi.VARIANT_MFMA
+    finally:
+        abi.plan_destroy(plan)
