@@ -425,6 +425,9 @@ PYBIND11_MODULE(_host, m) {
           return measure(c, n_qubits, spec.scalar_bytes(), budget_s, seed,
                          [&](const QuantumGate &g, double b) {
                            const KernelId id = mgr.generate(g);
+                           // Settle the per-gate kernel choice first, so the
+                           // timed samples all run the kernel bench runs.
+                           mgr.tune({id}, sv, 12);
                            return mgr.time_adaptive(id, sv, b).mean_s;
                          });
         },
