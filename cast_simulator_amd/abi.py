@@ -160,6 +160,8 @@ _SIGS = {
     "cast_hip_expect_zz": (
         _i, [_u64, _u32, _u8, C.POINTER(_u32), _sz, C.POINTER(C.c_double), _vp, _err, _sz]),
     "cast_hip_traj_run": (_i, [C.POINTER(TrajLaunch), _vp, _err, _sz]),
+    "cast_hip_inner_product": (
+        _i, [_u64, _u8, _u64, _u8, _u32, C.POINTER(C.c_double), _vp, _err, _sz]),
 }
 
 
@@ -378,6 +380,13 @@ def observe_async(dptr: int, n_qubits: int, precision: int, pairs, out: int, scr
     arr = (_u32 * max(1, len(flat)))(*flat)
     _call("cast_hip_observe_async", dptr, n_qubits, precision, arr, len(pairs), out, scratch,
           stream)
+
+
+def inner_product(a: int, prec_a: int, b: int, prec_b: int, n_qubits: int,
+                  stream: int | None = None) -> complex:
+    out = (C.c_double * 2)()
+    _call("cast_hip_inner_product", a, prec_a, b, prec_b, n_qubits, out, stream)
+    return complex(out[0], out[1])
 
 
 def traj_run(args: TrajLaunch, stream: int | None = None) -> None:

@@ -712,4 +712,19 @@ int cast_hip_observe_async(uint64_t sv_dptr, uint32_t sv_n_qubits, uint8_t preci
   });
 }
 
+int cast_hip_inner_product(uint64_t a, uint8_t precision_a, uint64_t b, uint8_t precision_b,
+                           uint32_t sv_n_qubits, double *out_re_im, void *stream, char *err,
+                           size_t len) {
+  return guard(err, len, [&] {
+    check_precision(precision_a);
+    check_precision(precision_b);
+    if (!a || !b || !out_re_im)
+      throw Error("null statevector or output pointer");
+    if (sv_n_qubits < 1 || sv_n_qubits > 40)
+      throw Error("statevector qubit count out of range");
+    inner_product(reinterpret_cast<const void *>(a), precision_a, reinterpret_cast<const void *>(b),
+                  precision_b, sv_n_qubits, out_re_im, static_cast<hipStream_t>(stream));
+  });
+}
+
 } // extern "C"

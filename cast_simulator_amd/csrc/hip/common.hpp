@@ -115,6 +115,10 @@ void launch_scale(void *sv, size_t n_elements, uint8_t precision, double scale,
 void expect_zz(const void *sv, uint32_t n_qubits, uint8_t precision, const uint32_t *pairs,
                size_t n_pairs, double *out, hipStream_t stream);
 
+// <a|b> in f64 over two statevectors of n qubits, any precisions; synchronous.
+void inner_product(const void *a, uint8_t prec_a, const void *b, uint8_t prec_b,
+                   uint32_t n_qubits, double *out, hipStream_t stream);
+
 // Deterministic f64 norm + pair sums into device `out` (sv_kernels.hip).
 size_t observe_scratch_bytes(uint32_t n_pairs);
 void launch_observe(const void *sv, uint32_t n_qubits, uint8_t precision, const uint32_t *pairs,

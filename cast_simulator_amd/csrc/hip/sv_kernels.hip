@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 namespace cast_hip {
@@ -169,6 +170,29 @@ __global__ void __launch_bounds__(kRedBlock) k_observe(const T *__restrict__ sv,
     }
 }
 
+// <a|b> = sum conj(a_j) b_j in f64, per-block partials (re, im) at
+// partial[2 * block + {0, 1}].
+template <typename TA, typename TB>
+__global__ void __launch_bounds__(kRedBlock) k_inner(const TA *__restrict__ a,
+                                                     const TB *__restrict__ b, uint64_t n_amp,
+                                                     double *__restrict__ partial) {
+  __shared__ double lds[kRedBlock / 64];
+  double re = 0.0, im = 0.0;
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; j < n_amp; j += stride) {
+    const double ar = double(a[2 * j]), ai = double(a[2 * j + 1]);
+    const double br = double(b[2 * j]), bi = double(b[2 * j + 1]);
+    re = __builtin_fma(ar, br, __builtin_fma(ai, bi, re));
+    im = __builtin_fma(ar, bi, __builtin_fma(-ai, br, im));
+  }
+  const double sr = block_sum(re, lds);
+  const double si = block_sum(im, lds);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = sr;
+    partial[2 * blockIdx.x + 1] = si;
+  }
+}
+
 unsigned red_grid(uint64_t work) {
   const uint64_t need = (work + kRedBlock - 1) / kRedBlock;
   const uint64_t cap = std::min<uint64_t>(kMaxRedGrid, uint64_t(device_props().n_cu) * 4);
@@ -301,6 +325,38 @@ void launch_observe(const void *sv, uint32_t n_qubits, uint8_t precision, const 
   hipLaunchKernelGGL(k_sum_partials, dim3(1 + pa.n_pairs), dim3(kRedBlock), 0, stream, scratch,
                      grid, 1 + pa.n_pairs, out);
   check(hipGetLastError(), "hipLaunchKernelGGL(k_sum_partials)");
+}
+
+void inner_product(const void *a, uint8_t prec_a, const void *b, uint8_t prec_b,
+                   uint32_t n_qubits, double *out, hipStream_t stream) {
+  RedScratch &s = scratch();
+  std::lock_guard<std::mutex> lock(s.mu);
+  s.ensure();
+  const uint64_t n_amp = uint64_t(1) << n_qubits;
+  const unsigned grid = red_grid(n_amp);
+  auto go = [&](auto *pa, auto *pb) {
+    using A = std::remove_const_t<std::remove_pointer_t<decltype(pa)>>;
+    using B = std::remove_const_t<std::remove_pointer_t<decltype(pb)>>;
+    hipLaunchKernelGGL((k_inner<A, B>), dim3(grid), dim3(kRedBlock), 0, stream, pa, pb, n_amp,
+                       s.d_partial);
+  };
+  if (prec_a == 0 && prec_b == 0)
+    go(static_cast<const float *>(a), static_cast<const float *>(b));
+  else if (prec_a == 0)
+    go(static_cast<const float *>(a), static_cast<const double *>(b));
+  else if (prec_b == 0)
+    go(static_cast<const double *>(a), static_cast<const float *>(b));
+  else
+    go(static_cast<const double *>(a), static_cast<const double *>(b));
+  check(hipGetLastError(), "hipLaunchKernelGGL(k_inner)");
+  hipLaunchKernelGGL(k_sum_partials, dim3(2), dim3(kRedBlock), 0, stream, s.d_partial, grid, 2u,
+                     s.d_out);
+  check(hipGetLastError(), "hipLaunchKernelGGL(k_sum_partials)");
+  check(hipMemcpyAsync(s.h_out, s.d_out, 2 * sizeof(double), hipMemcpyDeviceToHost, stream),
+        "hipMemcpyAsync(inner)");
+  check(hipStreamSynchronize(stream), "hipStreamSynchronize(inner)");
+  out[0] = s.h_out[0];
+  out[1] = s.h_out[1];
 }
 
 } // namespace cast_hip

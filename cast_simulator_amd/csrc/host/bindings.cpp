@@ -333,6 +333,8 @@ PYBIND11_MODULE(_host, m) {
       .def("norm_squared", &HipStatevector::norm_squared)
       .def("normalize", &HipStatevector::normalize)
       .def("clone_device", &HipStatevector::clone_device)
+      .def("inner", &HipStatevector::inner)
+      .def("fidelity", &HipStatevector::fidelity)
       .def("expect_zz", &HipStatevector::expect_zz);
 
   py::class_<PlanInfo>(m, "PlanInfo")

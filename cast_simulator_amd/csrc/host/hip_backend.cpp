@@ -93,6 +93,23 @@ std::vector<double> HipStatevector::expect_zz(const std::vector<uint32_t> &pairs
   return out;
 }
 
+cplx HipStatevector::inner(const HipStatevector &o) const {
+  if (o.n_ != n_)
+    throw Error("inner: statevectors have different qubit counts");
+  double out[2] = {0.0, 0.0};
+  char err[kErr];
+  throw_if(cast_hip_inner_product(dptr_, uint8_t(prec_), o.dptr_, uint8_t(o.prec_), n_, out,
+                                  nullptr, err, kErr),
+           err);
+  return {out[0], out[1]};
+}
+
+double HipStatevector::fidelity(const HipStatevector &o) const {
+  const cplx ab = inner(o);
+  const double aa = inner(*this).real(), bb = o.inner(o).real();
+  return std::norm(ab) / (aa * bb);
+}
+
 // ── HipKernelManager ──────────────────────────────────────────────────────
 
 namespace {

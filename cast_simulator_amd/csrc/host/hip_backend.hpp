@@ -64,6 +64,10 @@ public:
   void normalize();
   HipStatevector clone_device() const;
   std::vector<double> expect_zz(const std::vector<uint32_t> &pairs) const;
+  // <this|other> accumulated in f64 on the device (precisions may differ).
+  cplx inner(const HipStatevector &other) const;
+  // |<this|other>|^2 / (<this|this> <other|other>).
+  double fidelity(const HipStatevector &other) const;
 
 private:
   uint64_t dptr_ = 0;

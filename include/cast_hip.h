@@ -249,6 +249,15 @@ int cast_hip_observe_async(uint64_t sv_dptr, uint32_t sv_n_qubits, uint8_t preci
                            const uint32_t *pairs, size_t n_pairs, uint64_t out, uint64_t scratch,
                            void *stream, char *err_buf, size_t err_buf_len);
 
+/* <a|b> = sum_j conj(a_j) b_j over two statevectors of sv_n_qubits qubits,
+ * each F32 or F64, accumulated in f64; out_re_im = {Re, Im}.  Synchronous on
+ * `stream`.  Fidelity |<a|b>|^2 / (<a|a><b|b>) of a reduced-precision result
+ * against an f64 one without copying either to the host.  No reference
+ * counterpart. */
+int cast_hip_inner_product(uint64_t a, uint8_t precision_a, uint64_t b, uint8_t precision_b,
+                           uint32_t sv_n_qubits, double *out_re_im, void *stream,
+                           char *err_buf, size_t err_buf_len);
+
 /* ── Batched trajectories in one persistent launch ───────────────────────────
  * No reference counterpart (the reference branches deterministic ensembles
  * on the host, src/simulator/trajectory.rs:180-265).  Each workgroup owns

@@ -488,3 +488,24 @@ def test_mfma_k6_at_24_qubits_matches_direct_kernel(qubits, f32):
         assert abi.plan_family(plan, n) == abi.VARIANT_MFMA
     finally:
         abi.plan_destroy(plan)
+
+
+@pytest.mark.parametrize("pa,pb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_inner_product_and_fidelity_match_numpy(pa, pb):
+    n = 13
+    x = np.asarray(cs.random_statevector(n, 3))
+    y = np.asarray(cs.random_statevector(n, 4))
+    prec = {0: cs.Precision.F32, 1: cs.Precision.F64}
+    a, b = cs.HipStatevector(n, prec[pa]), cs.HipStatevector(n, prec[pb])
+    a.upload(x)
+    b.upload(y)
+    xa, yb = np.asarray(a.download()), np.asarray(b.download())  # rounded as stored
+    za, zb = xa[0::2] + 1j * xa[1::2], yb[0::2] + 1j * yb[1::2]
+    want = np.vdot(za, zb)
+    got = a.inner(b)
+    assert abs(got - want) < 1e-12
+    f = abs(want) ** 2 / (np.vdot(za, za).real * np.vdot(zb, zb).real)
+    assert abs(a.fidelity(b) - f) < 1e-12
+    assert abs(a.fidelity(a) - 1.0) < 1e-12
+    with pytest.raises(cs.CastError):
+        a.inner(cs.HipStatevector(n - 1, prec[pb]))
