@@ -177,6 +177,9 @@ HardwareProfile measure(ProfileConfig config, uint32_t n_qubits, size_t scalar_b
   Rng rng(seed);
   return adaptive_sweep(
       [&](double target_ai, double budget) {
+        // Refinement probes can ask for more than a dense 6-qubit gate
+        // reaches (AI 2 * 2^6); the backend's kernels stop at k = 6.
+        target_ai = std::min(target_ai, 128.0);
         const auto g = QuantumGate::random_arithmetic_intensity(n_qubits, target_ai, rng);
         const double ai = g.arithmetic_intensity(ztol);
         const double mean_s = time_kernel(g, budget);
