@@ -32,7 +32,8 @@ sys.path.insert(0, str(ROOT))
 METRIC = "HBM GB/s per fused gate at 30q f64; batched circuits/s + p99 at fixed fidelity"
 HBM_PEAK_GBS = 8000.0
 SEED = 20260101
-KERNEL = {1: "k_dense", 2: "k_monomial", 3: "k_tile", 4: "k_mfma", 5: "k_tile(swizzled)"}
+KERNEL = {1: "k_dense", 2: "k_monomial", 3: "k_tile", 4: "k_mfma", 5: "k_tile(swizzled)",
+          6: "k_tile(3 products)", 7: "k_tile(3 products, swizzled)"}
 
 
 def log(msg: str) -> None:
@@ -90,9 +91,9 @@ def run_gpu(cs, n: int, precision: str, steps: int, warmup: int, max_size: int):
     def symbol(kid):
         """Kernel symbol as the profiler names it."""
         k, f = info[kid].k, fam[kid]
-        if f in (3, 5):
+        if f in (3, 5, 6, 7):
             low = sum(1 for q in info[kid].qubits if q < 6)
-            return f"k_tile<{ctype}, {k}, {low}>"
+            return f"k_tile<{ctype}, {k}, {low}" + (", true>" if f in (6, 7) else ">")
         return f"{KERNEL[f]}<{ctype}, {k}>"
 
     per_kernel = defaultdict(list)

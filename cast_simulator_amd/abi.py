@@ -40,6 +40,7 @@ class KernelGenSpec(C.Structure):
 
 
 FLAG_TILE_SWIZZLE = 1
+FLAG_TILE_GAUSS = 2
 
 
 class PlanInfo(C.Structure):

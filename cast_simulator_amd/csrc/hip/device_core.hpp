@@ -128,7 +128,11 @@ __device__ __forceinline__ void dense_combo(const typename V2<T>::type *src,
 // outer loop and tiles the inner one, so one matrix row (2 K scalars, read by
 // scalar loads) is live at a time whatever U is.  Register arrays are local
 // and indexed by constants only, so they stay in VGPRs.
-template <typename T, int KL, int L, int U>
+//
+// G3 selects the three-product form: `mat` then holds three real K x K
+// matrices [Re M | Im M | Re M + Im M] and each output needs 3 K FMAs instead
+// of 4 K (Re = Mr vr - Mi vi, Im = (Mr + Mi)(vr + vi) - Mr vr - Mi vi).
+template <typename T, int KL, int L, int U, bool G3 = false>
 __device__ __forceinline__ void tile_step(const T *const *src, T *const *dst,
                                           const T *__restrict__ mat, const uint32_t *reg_pos,
                                           const uint32_t *low_t, uint32_t lane) {
@@ -165,6 +169,14 @@ __device__ __forceinline__ void tile_step(const T *const *src, T *const *dst,
         }
       }
   }
+  T vs[G3 ? U : 1][G3 ? K : 1];
+  if constexpr (G3) {
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int c = 0; c < K; ++c)
+        vs[u][c] = vr[u][c] + vi[u][c];
+  }
 #pragma unroll
   for (int h = 0; h < K / G; ++h) {
     T orr[U][G], oi[U][G];
@@ -173,19 +185,31 @@ __device__ __forceinline__ void tile_step(const T *const *src, T *const *dst,
       const int r = h * G + q;
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        // Four independent accumulators: dependent FMA chains of length K.
-        T ar0 = T(0), ar1 = T(0), ai0 = T(0), ai1 = T(0);
+        if constexpr (G3) {
+          T t1 = T(0), t2 = T(0), t3 = T(0);
 #pragma unroll
-        for (int c = 0; c < K; ++c) {
-          const T mr = mat[2 * (r * K + c)];
-          const T mi = mat[2 * (r * K + c) + 1];
-          ar0 = fma_t(mr, vr[u][c], ar0);
-          ar1 = fma_t(mi, vi[u][c], ar1);
-          ai0 = fma_t(mr, vi[u][c], ai0);
-          ai1 = fma_t(mi, vr[u][c], ai1);
+          for (int c = 0; c < K; ++c) {
+            t1 = fma_t(mat[r * K + c], vr[u][c], t1);
+            t2 = fma_t(mat[K * K + r * K + c], vi[u][c], t2);
+            t3 = fma_t(mat[2 * K * K + r * K + c], vs[u][c], t3);
+          }
+          orr[u][q] = t1 - t2;
+          oi[u][q] = t3 - t1 - t2;
+        } else {
+          // Four independent accumulators: dependent FMA chains of length K.
+          T ar0 = T(0), ar1 = T(0), ai0 = T(0), ai1 = T(0);
+#pragma unroll
+          for (int c = 0; c < K; ++c) {
+            const T mr = mat[2 * (r * K + c)];
+            const T mi = mat[2 * (r * K + c) + 1];
+            ar0 = fma_t(mr, vr[u][c], ar0);
+            ar1 = fma_t(mi, vi[u][c], ar1);
+            ai0 = fma_t(mr, vi[u][c], ai0);
+            ai1 = fma_t(mi, vr[u][c], ai1);
+          }
+          orr[u][q] = ar0 - ar1;
+          oi[u][q] = ai0 + ai1;
         }
-        orr[u][q] = ar0 - ar1;
-        oi[u][q] = ai0 + ai1;
       }
     }
 #pragma unroll
@@ -223,13 +247,13 @@ __device__ __forceinline__ void tile_step(const T *const *src, T *const *dst,
 }
 
 // One wave-tile update (src == dst for in place).
-template <typename T, int KL, int L>
+template <typename T, int KL, int L, bool G3 = false>
 __device__ __forceinline__ void tile_update(const T *src, T *dst, const T *__restrict__ mat,
                                             const uint32_t *reg_pos, const uint32_t *low_t,
                                             uint32_t lane) {
   const T *const s[1] = {src};
   T *const d[1] = {dst};
-  tile_step<T, KL, L, 1>(s, d, mat, reg_pos, low_t, lane);
+  tile_step<T, KL, L, 1, G3>(s, d, mat, reg_pos, low_t, lane);
 }
 
 // Wave-tile geometry of a gate with ascending targets q[0..k) on an n-qubit

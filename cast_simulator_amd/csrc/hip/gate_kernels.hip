@@ -78,7 +78,7 @@ struct TileGeom {
   uint32_t swz;        // 0, or the number of low tile-index bits visited bit-reversed
 };
 
-template <typename T, int KL, int L>
+template <typename T, int KL, int L, bool G3 = false>
 __global__ void __launch_bounds__(256) k_tile(T *__restrict__ sv, const T *__restrict__ mat,
                                               TileGeom g) {
   const uint32_t lane = threadIdx.x & 63u;
@@ -96,7 +96,7 @@ __global__ void __launch_bounds__(256) k_tile(T *__restrict__ sv, const T *__res
     tb = insert_zero(tb, g.ins[b]);
   T *__restrict__ p = sv + 2 * ((tb << 6) | lane);
 
-  dev::tile_update<T, KL, L>(p, p, mat, g.reg_pos, g.low_t, lane);
+  dev::tile_update<T, KL, L, G3>(p, p, mat, g.reg_pos, g.low_t, lane);
 }
 
 // ── Monomial: at most one nonzero per row ───────────────────────────────────
@@ -400,6 +400,15 @@ void launch_mfma6_t(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &
 template <typename T, int KL, int L>
 void launch_tile_kl(const GatePlan &p, hipStream_t s, void *sv, const TileGeom &tg) {
   const uint64_t blocks = (tg.n_tiles + 3) / 4; // 4 waves per block, one tile per wave
+  // Three-product form for k = 4, 5 when asked for (needs the real matrices).
+  if constexpr (KL == 4 || KL == 5) {
+    if ((p.flags & CAST_HIP_FLAG_TILE_GAUSS) && p.d_mat3) {
+      hipLaunchKernelGGL((k_tile<T, KL, L, true>), full_grid(blocks * kBlock), dim3(kBlock), 0, s,
+                         static_cast<T *>(sv), static_cast<const T *>(p.d_mat3), tg);
+      check(hipGetLastError(), "hipLaunchKernelGGL(k_tile, three products)");
+      return;
+    }
+  }
   hipLaunchKernelGGL((k_tile<T, KL, L>), full_grid(blocks * kBlock), dim3(kBlock), 0, s,
                      static_cast<T *>(sv), static_cast<const T *>(p.d_mat), tg);
   check(hipGetLastError(), "hipLaunchKernelGGL(k_tile)");

@@ -163,6 +163,8 @@ HipKernelManager::TuneState &HipKernelManager::tune_state_locked(KernelId id, ui
   const std::pair<uint32_t, uint32_t> cands[] = {
       {CAST_HIP_VARIANT_TILE, 0u},
       {CAST_HIP_VARIANT_TILE, CAST_HIP_FLAG_TILE_SWIZZLE},
+      {CAST_HIP_VARIANT_TILE, CAST_HIP_FLAG_TILE_GAUSS},
+      {CAST_HIP_VARIANT_TILE, CAST_HIP_FLAG_TILE_GAUSS | CAST_HIP_FLAG_TILE_SWIZZLE},
       {CAST_HIP_VARIANT_MFMA, 0u},
       {CAST_HIP_VARIANT_DENSE, 0u}};
   for (const auto &c : cands) {
@@ -178,9 +180,14 @@ HipKernelManager::TuneState &HipKernelManager::tune_state_locked(KernelId id, ui
              err);
     uint32_t fam = 0;
     const bool ok = cast_hip_gate_plan_family(plan, n, &fam, err, kErr) == 0;
-    // Candidate code: the family, or kTileSwizzled for the swizzled wave tiles.
-    const uint32_t code = (fam == CAST_HIP_VARIANT_TILE && c.second) ? kTileSwizzled : fam;
-    if (!ok || (c.second && fam != CAST_HIP_VARIANT_TILE) ||
+    // Candidate code: the family, or one of the wave-tile variants.
+    const bool gauss = c.second & CAST_HIP_FLAG_TILE_GAUSS;
+    const bool swz = c.second & CAST_HIP_FLAG_TILE_SWIZZLE;
+    uint32_t code = fam;
+    if (fam == CAST_HIP_VARIANT_TILE && c.second)
+      code = gauss ? (swz ? kTileGaussSwizzled : kTileGauss) : kTileSwizzled;
+    const bool gauss_ok = g.n_qubits() == 4 || g.n_qubits() == 5;
+    if (!ok || (c.second && fam != CAST_HIP_VARIANT_TILE) || (gauss && !gauss_ok) ||
         std::find(st.fams.begin(), st.fams.end(), code) != st.fams.end()) {
       cast_hip_gate_plan_destroy(plan); // inapplicable here, or same kernel as another
       continue;

@@ -101,6 +101,10 @@ struct PlanInfo {
 // Candidate code of the wave-tile kernel visiting tiles in swizzled order
 // (the other codes are the CAST_HIP_VARIANT_* families).
 constexpr uint32_t kTileSwizzled = 5;
+// ... and of the wave tiles in the three-product form (CAST_HIP_FLAG_TILE_GAUSS),
+// in natural and swizzled order.
+constexpr uint32_t kTileGauss = 6;
+constexpr uint32_t kTileGaussSwizzled = 7;
 
 struct TuneEntry {
   KernelId id = 0;

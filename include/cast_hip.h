@@ -73,6 +73,10 @@ typedef struct cast_hip_kernel_gen_spec_t {
  * order streams faster depends on how the gate's target bits fall on the
  * HBM channel/bank mapping; the host autotuner tries both. */
 #define CAST_HIP_FLAG_TILE_SWIZZLE 1u
+/* Wave tiles of k = 4, 5 in the three-product form (3 K instead of 4 K FMAs
+ * per output: Re = Mr vr - Mi vi, Im = (Mr + Mi)(vr + vi) - Mr vr - Mi vi),
+ * for gates where the FMA rate, not HBM, limits the wave tiles. */
+#define CAST_HIP_FLAG_TILE_GAUSS 2u
 
 /* AUTO: monomial when every row has <= 1 nonzero, else the dense family, whose
  * kernel (direct / wave tile / matrix core) is picked per launch from k, the

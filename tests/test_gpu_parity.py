@@ -188,11 +188,14 @@ def apply_abi(matrix, qubits, init, f32, variant, n, flags=0):
 
 
 VARIANTS = [(abi.VARIANT_AUTO, 0), (abi.VARIANT_DENSE, 0), (abi.VARIANT_TILE, 0),
-            (abi.VARIANT_TILE, abi.FLAG_TILE_SWIZZLE), (abi.VARIANT_MFMA, 0)]
+            (abi.VARIANT_TILE, abi.FLAG_TILE_SWIZZLE), (abi.VARIANT_TILE, abi.FLAG_TILE_GAUSS),
+            (abi.VARIANT_TILE, abi.FLAG_TILE_GAUSS | abi.FLAG_TILE_SWIZZLE),
+            (abi.VARIANT_MFMA, 0)]
 
 
 @pytest.mark.parametrize("variant,flags", VARIANTS,
-                         ids=["auto", "direct", "tile", "tile-swizzled", "mfma"])
+                         ids=["auto", "direct", "tile", "tile-swizzled", "tile-gauss",
+                              "tile-gauss-swizzled", "mfma"])
 @pytest.mark.parametrize("f32", [False, True], ids=["f64", "f32"])
 def test_every_kernel_family_every_shape(f32, variant, flags):
     """Each dense kernel family, forced through the C ABI, on every target
@@ -435,7 +438,7 @@ def test_autotuned_kernels_match_oracle(f32):
     assert mgr.tune(ids, scratch) >= 1
     report = mgr.tuning_report()
     # chosen: direct (1), wave tile (3), matrix core (4), swizzled wave tile (5)
-    assert len(report) == len(ids) and all(e["chosen"] in (1, 3, 4, 5) for e in report)
+    assert len(report) == len(ids) and all(e["chosen"] in (1, 3, 4, 5, 6, 7) for e in report)
     init = np.asarray(cs.random_statevector(n, 3))
     sv = cs.HipStatevector(n, prec(f32))
     sv.upload(init)
