@@ -93,7 +93,8 @@ def run_gpu(cs, n: int, precision: str, steps: int, warmup: int, max_size: int):
         k, f = info[kid].k, fam[kid]
         if f in (3, 5, 6, 7):
             low = sum(1 for q in info[kid].qubits if q < 6)
-            return f"k_tile<{ctype}, {k}, {low}" + (", true>" if f in (6, 7) else ">")
+            return (f"k_tile3<{ctype}, {k}, {low}>" if f in (6, 7)
+                    else f"k_tile<{ctype}, {k}, {low}>")
         return f"{KERNEL[f]}<{ctype}, {k}>"
 
     per_kernel = defaultdict(list)

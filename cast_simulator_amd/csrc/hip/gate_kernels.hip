@@ -78,7 +78,7 @@ struct TileGeom {
   uint32_t swz;        // 0, or the number of low tile-index bits visited bit-reversed
 };
 
-template <typename T, int KL, int L, bool G3 = false>
+template <typename T, int KL, int L>
 __global__ void __launch_bounds__(256) k_tile(T *__restrict__ sv, const T *__restrict__ mat,
                                               TileGeom g) {
   const uint32_t lane = threadIdx.x & 63u;
@@ -96,7 +96,31 @@ __global__ void __launch_bounds__(256) k_tile(T *__restrict__ sv, const T *__res
     tb = insert_zero(tb, g.ins[b]);
   T *__restrict__ p = sv + 2 * ((tb << 6) | lane);
 
-  dev::tile_update<T, KL, L, G3>(p, p, mat, g.reg_pos, g.low_t, lane);
+  dev::tile_update<T, KL, L>(p, p, mat, g.reg_pos, g.low_t, lane);
+}
+
+// Three-product wave tiles (k = 4, 5): the same tile update with the real
+// matrices [Re | Im | Re + Im].  At f64 and k = 5 the third input vector
+// pushes the update past 256 VGPRs; the occupancy bound (two workgroups of
+// four waves per CU, i.e. two waves per SIMD) keeps it at 256.
+template <typename T, int KL, int L>
+__global__ void __launch_bounds__(256, 2) k_tile3(T *__restrict__ sv, const T *__restrict__ m3,
+                                                  TileGeom g) {
+  const uint32_t lane = threadIdx.x & 63u;
+  uint64_t tile =
+      ((uint64_t(blockIdx.y) * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+  if (tile >= g.n_tiles)
+    return; // whole wave
+  if (g.swz) {
+    const uint64_t lo = tile & ((uint64_t(1) << g.swz) - 1);
+    tile = (tile ^ lo) | (__brevll(lo) >> (64 - g.swz));
+  }
+  uint64_t tb = tile;
+#pragma unroll
+  for (int b = 0; b < KL; ++b)
+    tb = insert_zero(tb, g.ins[b]);
+  T *__restrict__ p = sv + 2 * ((tb << 6) | lane);
+  dev::tile_update<T, KL, L, true>(p, p, m3, g.reg_pos, g.low_t, lane);
 }
 
 // ── Monomial: at most one nonzero per row ───────────────────────────────────
@@ -403,7 +427,7 @@ void launch_tile_kl(const GatePlan &p, hipStream_t s, void *sv, const TileGeom &
   // Three-product form for k = 4, 5 when asked for (needs the real matrices).
   if constexpr (KL == 4 || KL == 5) {
     if ((p.flags & CAST_HIP_FLAG_TILE_GAUSS) && p.d_mat3) {
-      hipLaunchKernelGGL((k_tile<T, KL, L, true>), full_grid(blocks * kBlock), dim3(kBlock), 0, s,
+      hipLaunchKernelGGL((k_tile3<T, KL, L>), full_grid(blocks * kBlock), dim3(kBlock), 0, s,
                          static_cast<T *>(sv), static_cast<const T *>(p.d_mat3), tg);
       check(hipGetLastError(), "hipLaunchKernelGGL(k_tile, three products)");
       return;
