@@ -375,6 +375,24 @@ def run_serving(cs, count: int, rate: float, capacity_sample: int):
     }
 
 
+def run_trajectory_launch(cs, n: int, count: int, tuning):
+    """The trajectory workload of run_trajectories with a fixed tuning and no
+    plain-path comparison: one plan + checkpoints + persistent launch."""
+    D, p = 40, 1e-3
+    lc = cs.LayeredCircuit.random_layers(n, D, SEED)
+    pairs = [q for i in range(n - 1) for q in (i, i + 1)]
+    S = cs.sample_trajectories(lc, cs.NoiseSpec(p, 10, D, SEED), count)
+    eng = cs.TrajectoryEngine(lc, cs.Precision.F32, pairs)
+    r = eng.run(S, tuning)
+    sv_bytes = (2 << n) * 4
+    alg = r.n_gate_passes * 2 * sv_bytes + r.n_tasks * sv_bytes
+    return {"workload": f"{count} trajectories of the {n}-qubit 40-layer circuit, f32",
+            "tuning": repr(tuning), "tasks": r.n_tasks, "gate_passes": r.n_gate_passes,
+            "slots": r.n_slots, "gpu_s": r.gpu_s, "total_s": r.total_s,
+            "circuits_per_s": count / r.total_s, "algorithmic_bytes": alg,
+            "kernel_gbs": alg / r.gpu_s / 1e9}
+
+
 def cpu_baseline(gates, n_gpu: int, budget_s: float = 20.0):
     """The oracle's SIMD split-layout kernels (a restatement of the
     reference CPU path) on this host's logical CPUs, over the first fused
@@ -427,9 +445,13 @@ def main() -> int:
     ap.add_argument("--max-size", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f32", action="store_true")
-    ap.add_argument("--workload", choices=["headline", "dense-large"], default="headline",
-                    help="dense-large: the 33q f64 / 34q f32 k<=6 matrix-core workload "
-                         "(its own JSON line)")
+    ap.add_argument("--workload", choices=["headline", "dense-large", "trajectories"],
+                    default="headline",
+                    help="dense-large: the 33q f64 / 34q f32 k<=6 matrix-core workload; "
+                         "trajectories: the batched-trajectory launch alone with a fixed "
+                         "tuning (for profiling); each prints its own JSON line")
+    ap.add_argument("--traj-tuning", default="8,4,512,3",
+                    help="chunk_layers,fusion_max,block,slots_per_cu for --workload trajectories")
     ap.add_argument("--no-traj", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--requests", type=int, default=1000)
@@ -449,6 +471,12 @@ def main() -> int:
 
     if args.workload == "dense-large":
         print(json.dumps(run_dense_large(cs)), flush=True)
+        return 0
+    if args.workload == "trajectories":
+        b, k, blk, spc = (int(x) for x in args.traj_tuning.split(","))
+        print(json.dumps(run_trajectory_launch(cs, args.traj_qubits, args.trajectories,
+                                               cs.TrajectoryTuning(b, k, blk, spc, 0))),
+              flush=True)
         return 0
 
     if args.copy_probe_qubits > 0:

@@ -2,12 +2,20 @@
 
     python -m cast_simulator_amd.rocprof_summary kernel_trace.csv [--last N]
     python -m cast_simulator_amd.rocprof_summary counter_collection.csv --counters
+    python -m cast_simulator_amd.rocprof_summary FETCH.csv --traffic WRITE.csv \
+        --copy-bytes B --last N --algorithmic A
 
 Kernel-trace mode prints per-kernel dispatch statistics (count, mean, min,
 max, total) over all dispatches, and, with --last N, over the last N
 dispatches of gate kernels only (the timed step of a bench run, which comes
 after its autotuning passes).  Counter mode prints the per-kernel mean of
 each collected counter over dispatches.
+
+Traffic mode turns one FETCH_SIZE run and one WRITE_SIZE run (KiB per
+dispatch) of the same program into HBM bytes per launch of each gate kernel
+over the last N gate dispatches, calibrated by what the same counters read
+for the device-to-device copy of B bytes the program makes first (the
+`--copy-probe-qubits` option of bench.py): factor = counter bytes / B.
 """
 from __future__ import annotations
 
@@ -71,13 +79,61 @@ def counters(path: str) -> str:
     return "\n".join(out) + "\n"
 
 
+def _gate_and_copy(path: str):
+    with open(path, newline="") as f:
+        rows = list(csv.DictReader(f))
+    rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+    copy = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"]]
+    gate = [(_short(r["Kernel_Name"]), float(r["Counter_Value"])) for r in rows
+            if _short(r["Kernel_Name"]).startswith(GATE_KERNELS)]
+    return max(copy) if copy else None, gate
+
+
+def traffic(fetch_csv: str, write_csv: str, copy_bytes: int, last: int, algorithmic: float):
+    import json
+
+    fcopy, fgate = _gate_and_copy(fetch_csv)
+    wcopy, wgate = _gate_and_copy(write_csv)
+    if not fcopy or not wcopy:
+        raise SystemExit("no copy-probe dispatch found in the counter files")
+    ff, wf = fcopy * 1024 / copy_bytes, wcopy * 1024 / copy_bytes
+    # Per kernel symbol, each run on its own: the autotuner may settle a gate
+    # on a different family in the two runs, so dispatches are not paired.
+    def means(sel, factor):
+        acc = defaultdict(list)
+        for name, v in sel:
+            acc[name].append(v * 1024 / factor)
+        return {n: sum(v) / len(v) for n, v in acc.items()}, {n: len(v) for n, v in acc.items()}
+
+    rd, rn = means(fgate[-last:], ff)
+    wr, wn = means(wgate[-last:], wf)
+    both = [n for n in rd if n in wr]
+    out = {
+        "calibration": {"copy_bytes_each_way": copy_bytes, "fetch_raw_kib": fcopy,
+                        "write_raw_kib": wcopy, "fetch_factor": ff, "write_factor": wf},
+        "algorithmic_bytes_per_launch": algorithmic,
+        "hbm_bytes_per_launch": {n: rd[n] + wr[n] for n in both},
+        "read_bytes_per_launch": {n: rd[n] for n in both},
+        "write_bytes_per_launch": {n: wr[n] for n in both},
+        "launches_fetch_run": {n: rn[n] for n in both},
+        "launches_write_run": {n: wn[n] for n in both},
+    }
+    return json.dumps(out, indent=1)
+
+
 def main() -> None:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("csv")
     ap.add_argument("--last", type=int, default=None)
     ap.add_argument("--counters", action="store_true")
+    ap.add_argument("--traffic", metavar="WRITE_CSV", default=None)
+    ap.add_argument("--copy-bytes", type=int, default=0)
+    ap.add_argument("--algorithmic", type=float, default=0.0)
     a = ap.parse_args()
-    print(counters(a.csv) if a.counters else kernel_trace(a.csv, a.last))
+    if a.traffic:
+        print(traffic(a.csv, a.traffic, a.copy_bytes, a.last or 0, a.algorithmic))
+    else:
+        print(counters(a.csv) if a.counters else kernel_trace(a.csv, a.last))
 
 
 if __name__ == "__main__":
