@@ -469,16 +469,6 @@ def main() -> int:
 
     import cast_simulator_amd as cs
 
-    if args.workload == "dense-large":
-        print(json.dumps(run_dense_large(cs)), flush=True)
-        return 0
-    if args.workload == "trajectories":
-        b, k, blk, spc = (int(x) for x in args.traj_tuning.split(","))
-        print(json.dumps(run_trajectory_launch(cs, args.traj_qubits, args.trajectories,
-                                               cs.TrajectoryTuning(b, k, blk, spc, 0))),
-              flush=True)
-        return 0
-
     if args.copy_probe_qubits > 0:
         from cast_simulator_amd import abi
 
@@ -490,6 +480,16 @@ def main() -> int:
         abi.sv_free(src)
         abi.sv_free(dst)
         log(f"[bench] copy probe: {nbytes} bytes device-to-device")
+
+    if args.workload == "dense-large":
+        print(json.dumps(run_dense_large(cs)), flush=True)
+        return 0
+    if args.workload == "trajectories":
+        b, k, blk, spc = (int(x) for x in args.traj_tuning.split(","))
+        print(json.dumps(run_trajectory_launch(cs, args.traj_qubits, args.trajectories,
+                                               cs.TrajectoryTuning(b, k, blk, spc, 0))),
+              flush=True)
+        return 0
 
     log(f"[bench] f64 {args.qubits}q circuit: warmup {args.warmup}, steps {args.steps}")
     r64 = run_gpu(cs, args.qubits, "f64", args.steps, args.warmup, args.max_size)
