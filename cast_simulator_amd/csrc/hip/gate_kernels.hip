@@ -348,6 +348,8 @@ __global__ void __launch_bounds__(256) k_dense_batch(BatchArgs a) {
 
 constexpr unsigned kBlock = 256;
 
+bool tile_gauss(const GatePlan &p); // defined with the family rule below
+
 unsigned grid_for(uint64_t combos) {
   static const unsigned cap = [] {
     const char *s = std::getenv("CAST_HIP_GRID_PER_CU");
@@ -426,7 +428,7 @@ void launch_tile_kl(const GatePlan &p, hipStream_t s, void *sv, const TileGeom &
   const uint64_t blocks = (tg.n_tiles + 3) / 4; // 4 waves per block, one tile per wave
   // Three-product form for k = 4, 5 when asked for (needs the real matrices).
   if constexpr (KL == 4 || KL == 5) {
-    if ((p.flags & CAST_HIP_FLAG_TILE_GAUSS) && p.d_mat3) {
+    if (tile_gauss(p)) {
       hipLaunchKernelGGL((k_tile3<T, KL, L>), full_grid(blocks * kBlock), dim3(kBlock), 0, s,
                          static_cast<T *>(sv), static_cast<const T *>(p.d_mat3), tg);
       check(hipGetLastError(), "hipLaunchKernelGGL(k_tile, three products)");
@@ -509,34 +511,55 @@ void launch_tile(const GatePlan &p, hipStream_t s, void *sv, uint32_t n) {
 // Kernel family for a dense plan on an n-qubit statevector.  A forced family
 // falls back to the direct kernel where its shape requirement is not met
 // (wave tiles need n >= k + 6, matrix-core tiles k in {4, 5, 6} and n >= k + 4).
-uint32_t dense_family(const GatePlan &p, uint32_t n) {
-  const bool tile_ok = p.k <= 5 && n >= p.k + 6;
-  const bool mfma_ok = p.d_mat3 != nullptr && n >= p.k + 4;
+// Whether an automatic choice may pick the family (variant AUTO, or DENSE
+// not forced).
+bool auto_choice(const GatePlan &p) { return p.variant == 0 || (p.variant == 1 && !p.forced); }
+
+uint32_t count_low(const GatePlan &p) {
   uint32_t n_low = 0;
   for (uint32_t b = 0; b < p.k; ++b)
     n_low += p.q[b] < 6;
+  return n_low;
+}
+
+uint32_t dense_family(const GatePlan &p, uint32_t n) {
+  const bool tile_ok = p.k <= 5 && n >= p.k + 6;
+  const bool mfma_ok = p.d_mat3 != nullptr && n >= p.k + 4;
+  const uint32_t n_low = count_low(p);
   uint32_t v = p.variant;
-  if (v == 0 || (v == 1 && !p.forced)) {
-    // Automatic choice, from 30-qubit measurements on MI355X:
-    //  * f32, k >= 4: the VALU kernels are FMA-bound without packed FP32;
-    //    the matrix-core three-product kernel is not.
-    //  * wave tiles stream full 1 KiB rows and win for up to 3 low-order
-    //    targets; with 4 or 5 the exchanged register groups cost more than
-    //    the matrix-core kernel's 256-byte segments.
+  if (auto_choice(p)) {
+    // Automatic choice, from 30-qubit measurements on MI355X (the per-gate
+    // autotuner refines it where it runs):
+    //  * wave tiles stream full rows and win up to 3 low-order targets for
+    //    k <= 4; beyond that the exchanged register groups cost more than
+    //    the matrix-core kernel's 16-combo tiles;
+    //  * k = 5: wave tiles only without low targets at f32 (FMA bound
+    //    otherwise) and with at most one at f64;
     //  * k = 6: only the matrix-core kernel keeps its operands out of
     //    scratch memory.
-    if (mfma_ok && (p.precision == 0 || n_low >= 4 || p.k == 6))
-      v = 4;
-    else if (tile_ok && n_low <= 3)
+    const uint32_t tile_low = p.k <= 4 ? 3 : (p.precision == 0 ? 0 : 1);
+    if (tile_ok && n_low <= tile_low)
       v = 3;
+    else if (mfma_ok)
+      v = 4;
     else
-      v = 1;
+      v = tile_ok ? 3 : 1;
   }
   if (v == 3 && !tile_ok)
     v = 1;
   if (v == 4 && !mfma_ok)
     v = tile_ok ? 3 : 1;
   return v;
+}
+
+// Three-product form of the wave tiles: asked for by flag, or chosen
+// automatically where it measured faster (k = 4; f32 k = 5).
+bool tile_gauss(const GatePlan &p) {
+  if (!p.d_mat3 || (p.k != 4 && p.k != 5))
+    return false;
+  if (p.flags & CAST_HIP_FLAG_TILE_GAUSS)
+    return true;
+  return auto_choice(p) && (p.k == 4 || p.precision == 0);
 }
 
 template <typename T>
