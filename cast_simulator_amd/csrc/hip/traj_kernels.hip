@@ -108,13 +108,13 @@ __device__ __forceinline__ void wg_tile_gate(const T *src, T *dst, const T *__re
         s[u] = src + off;
         d[u] = dst + off;
       }
-      dev::tile_step<T, KL, L, U>(s, d, m, reg_pos, low_t, lane);
+      dev::tile_step<T, KL, L, U, KL == 4>(s, d, m, reg_pos, low_t, lane);
     }
   } else {
     for (uint64_t t = wave; t < n_tiles; t += n_waves) {
       const T *m = mat + opaque_u32(0);
       const uint64_t off = tile_ptr(t);
-      dev::tile_update<T, KL, L>(src + off, dst + off, m, reg_pos, low_t, lane);
+      dev::tile_update<T, KL, L, KL == 4>(src + off, dst + off, m, reg_pos, low_t, lane);
     }
   }
 }
@@ -132,8 +132,34 @@ __device__ __forceinline__ void wg_dense_gate(const T *src, T *dst, const T *__r
   const uint64_t combos = uint64_t(1) << (n - KL);
   for (uint64_t c = threadIdx.x; c < combos; c += blockDim.x) {
     const T *m = mat + opaque_u32(0);
-    dev::dense_combo<T, KL>(reinterpret_cast<const VT *>(src), reinterpret_cast<VT *>(dst), m,
-                            combo_base<KL>(c, q), delta);
+    const VT *in = reinterpret_cast<const VT *>(src);
+    VT *out = reinterpret_cast<VT *>(dst);
+    const uint64_t base = combo_base<KL>(c, q);
+    if constexpr (KL == 4) { // three real matrices [Re | Im | Re + Im]
+      T vr[K], vi[K];
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        const VT x = in[base + delta[i]];
+        vr[i] = x.x;
+        vi[i] = x.y;
+      }
+#pragma unroll
+      for (int r = 0; r < K; ++r) {
+        T t1 = T(0), t2 = T(0), t3 = T(0);
+#pragma unroll
+        for (int cc = 0; cc < K; ++cc) {
+          t1 = dev::fma_t(m[r * K + cc], vr[cc], t1);
+          t2 = dev::fma_t(m[K * K + r * K + cc], vi[cc], t2);
+          t3 = dev::fma_t(m[2 * K * K + r * K + cc], vr[cc] + vi[cc], t3);
+        }
+        VT o;
+        o.x = t1 - t2;
+        o.y = t3 - t1 - t2;
+        out[base + delta[r]] = o;
+      }
+    } else {
+      dev::dense_combo<T, KL>(in, out, m, base, delta);
+    }
   }
 }
 
@@ -315,7 +341,9 @@ __global__ void __launch_bounds__(kMaxBlock)
         if (b < k)
           ok = ok && q[b] < n && (b == 0 || q[b] > q[b - 1]);
       }
-      ok = ok && uint64_t(moff) + (uint64_t(1) << (2 * k)) <= a.n_matrix_elems;
+      // k = 4 entries hold three real K x K matrices: 3 K^2 / 2 complex slots.
+      const uint64_t slots = k == 4 ? 384 : (uint64_t(1) << (2 * k));
+      ok = ok && uint64_t(moff) + slots <= a.n_matrix_elems;
       if (!ok) {
         atomicOr(&work[1], 1u);
         continue;

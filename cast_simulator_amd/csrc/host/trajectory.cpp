@@ -252,15 +252,29 @@ TrajectoryPlan TrajectoryEngine::plan(const std::vector<TrajectorySample> &sampl
   auto &tasks = P.tasks;
   auto &pool = P.pool;
   std::map<const Segment *, uint32_t> seg_pool; // clean matrices of a segment in this plan
+  // k <= 3: interleaved complex; k = 4: three real matrices Re, Im, Re + Im
+  // (the kernel's three-product form); both in whole complex slots.
   auto push_matrix = [&](const ComplexSquareMatrix &m) {
     const uint64_t off = pool.size() / 2;
-    if (off + m.data().size() > 0xffffffffull)
+    if (off + m.data().size() * 2 > 0xffffffffull)
       throw Error("trajectory matrix pool exceeds 2^32 complex entries");
-    for (const cplx &z : m.data()) {
-      pool.push_back(z.real());
-      pool.push_back(z.imag());
+    if (m.edge_size() == 16) {
+      for (const cplx &z : m.data())
+        pool.push_back(z.real());
+      for (const cplx &z : m.data())
+        pool.push_back(z.imag());
+      for (const cplx &z : m.data())
+        pool.push_back(z.real() + z.imag());
+    } else {
+      for (const cplx &z : m.data()) {
+        pool.push_back(z.real());
+        pool.push_back(z.imag());
+      }
     }
     return uint32_t(off);
+  };
+  auto slots_of = [](const ComplexSquareMatrix &m) {
+    return uint32_t(m.edge_size() == 16 ? 384 : m.data().size());
   };
   auto seg_offsets = [&](const Segment &seg) {
     auto it = seg_pool.find(&seg);
@@ -303,7 +317,7 @@ TrajectoryPlan TrajectoryEngine::plan(const std::vector<TrajectorySample> &sampl
         push_gate(f, push_matrix(dirty_matrix(f, it->second)));
         ++P.n_dirty_matrices;
       }
-      off += uint32_t(f.m.data().size());
+      off += slots_of(f.m);
     }
   };
 

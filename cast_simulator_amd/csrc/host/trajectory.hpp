@@ -100,7 +100,7 @@ struct TrajectoryRun {
 struct TrajectoryPlan {
   std::vector<cast_hip_traj_task_t> tasks;
   std::vector<cast_hip_traj_gate_t> gates;
-  std::vector<double> pool;      // interleaved complex
+  std::vector<double> pool;      // k <= 3: interleaved complex K x K; k = 4: Re, Im, Re + Im
   std::vector<uint32_t> order;   // visit order: longest task first
   std::vector<uint32_t> task_of; // per trajectory
   uint64_t n_shared_clean = 0, n_dirty_matrices = 0;

@@ -139,7 +139,13 @@ def test_plan_task_gate_lists_match_explicit_trajectories(B, kmax):
         for off, q in P["gates"][first:first + cnt]:
             assert len(q) <= max(kmax, 2) and list(q) == sorted(q)  # CZ stays 2-qubit
             K = 1 << len(q)
-            m = pool[off:off + K * K].reshape(K, K)
+            if K == 16:  # three real matrices: Re, Im, Re + Im
+                reals = pool.view(np.float64)[2 * off:2 * off + 3 * K * K]
+                re, im, rs = reals[:K * K], reals[K * K:2 * K * K], reals[2 * K * K:]
+                assert np.array_equal(rs, re + im)
+                m = (re + 1j * im).reshape(K, K)
+            else:
+                m = pool[off:off + K * K].reshape(K, K)
             oracle.apply_gate(got, m, list(q))
         assert rel_l2(want, got) < 1e-12, f"trajectory {t}"
         assert c * B <= s.depth
