@@ -300,7 +300,8 @@ def run_trajectories(cs, n: int, count: int, tune_sample: int, plain_sample: int
         "dirty_matrices": r.n_dirty_matrices,
         "slots": r.n_slots,
         "tuning": {"chunk_layers": best.chunk_layers, "fusion_max": best.fusion_max,
-                   "block": best.block, "slots_per_cu": best.slots_per_cu},
+                   "block": best.block, "slots_per_cu": best.slots_per_cu,
+                   "waves_per_simd": best.waves_per_simd},
         "autotune": {"trials": len(trials), "sample": tune_sample, "seconds": round(t_tune, 1)},
         "kernel_gbs": round(alg / r.gpu_s / 1e9, 1),
         "kernel_hbm_frac": round(alg / r.gpu_s / 1e9 / HBM_PEAK_GBS, 4),
@@ -450,8 +451,9 @@ def main() -> int:
                     help="dense-large: the 33q f64 / 34q f32 k<=6 matrix-core workload; "
                          "trajectories: the batched-trajectory launch alone with a fixed "
                          "tuning (for profiling); each prints its own JSON line")
-    ap.add_argument("--traj-tuning", default="8,4,512,3",
-                    help="chunk_layers,fusion_max,block,slots_per_cu for --workload trajectories")
+    ap.add_argument("--traj-tuning", default="4,4,512,1,2",
+                    help="chunk_layers,fusion_max,block,slots_per_cu,waves_per_simd for "
+                         "--workload trajectories")
     ap.add_argument("--no-traj", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--requests", type=int, default=1000)
@@ -485,9 +487,9 @@ def main() -> int:
         print(json.dumps(run_dense_large(cs)), flush=True)
         return 0
     if args.workload == "trajectories":
-        b, k, blk, spc = (int(x) for x in args.traj_tuning.split(","))
+        b, k, blk, spc, occ = (int(x) for x in args.traj_tuning.split(","))
         print(json.dumps(run_trajectory_launch(cs, args.traj_qubits, args.trajectories,
-                                               cs.TrajectoryTuning(b, k, blk, spc, 0))),
+                                               cs.TrajectoryTuning(b, k, blk, spc, 0, occ))),
               flush=True)
         return 0
 

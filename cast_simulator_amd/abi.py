@@ -109,6 +109,8 @@ class TrajLaunch(C.Structure):
         ("out", C.c_uint64),
         ("dump", C.c_uint64),
         ("work", C.c_uint64),
+        ("waves_per_simd", C.c_uint32),
+        ("reserved", C.c_uint32),
     ]
 
 

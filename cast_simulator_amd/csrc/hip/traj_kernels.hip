@@ -38,7 +38,7 @@ constexpr uint32_t kMaxTrajGateQubits = 4;
 // Layouts the host bindings rely on (no padding surprises across compilers).
 static_assert(sizeof(cast_hip_traj_gate_t) == 12, "cast_hip_traj_gate_t layout");
 static_assert(sizeof(cast_hip_traj_task_t) == 16, "cast_hip_traj_task_t layout");
-static_assert(sizeof(cast_hip_traj_launch_t) == 368, "cast_hip_traj_launch_t layout");
+static_assert(sizeof(cast_hip_traj_launch_t) == 376, "cast_hip_traj_launch_t layout");
 
 struct TrajScalars {
   uint64_t n_matrix_elems;
@@ -284,8 +284,11 @@ __device__ __forceinline__ void wg_observe(const T *sv, uint32_t n, uint32_t n_p
   __syncthreads();
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kMaxBlock)
+// Built for OCC waves per SIMD (second launch-bound argument): 2 allows 256
+// VGPRs (no spills, one 512-thread workgroup per CU), 4 allows 128 (spills,
+// but two workgroups per CU keep twice the loads in flight).
+template <typename T, int OCC>
+__global__ void __launch_bounds__(kMaxBlock, OCC)
     k_traj(const cast_hip_traj_task_t *__restrict__ tasks, const uint32_t *__restrict__ order,
            const cast_hip_traj_gate_t *__restrict__ gates, const T *__restrict__ matrices,
            const T *__restrict__ checkpoints, T *__restrict__ slots, double *__restrict__ out,
@@ -392,9 +395,9 @@ __global__ void __launch_bounds__(kMaxBlock)
   }
 }
 
-template <typename T>
+template <typename T, int OCC>
 void launch_traj_t(const cast_hip_traj_launch_t &L, const TrajScalars &a, hipStream_t s) {
-  hipLaunchKernelGGL((k_traj<T>), dim3(L.n_slots), dim3(L.block), 0, s,
+  hipLaunchKernelGGL((k_traj<T, OCC>), dim3(L.n_slots), dim3(L.block), 0, s,
                      reinterpret_cast<const cast_hip_traj_task_t *>(L.tasks),
                      reinterpret_cast<const uint32_t *>(L.order),
                      reinterpret_cast<const cast_hip_traj_gate_t *>(L.gates),
@@ -418,6 +421,8 @@ void launch_traj(const cast_hip_traj_launch_t &L, hipStream_t stream) {
     throw Error("trajectory launch: block must be 64, 128, 256 or 512");
   if (L.n_pairs > kMaxPairs)
     throw Error("trajectory launch: at most 32 qubit pairs");
+  if (L.waves_per_simd != 0 && L.waves_per_simd != 2 && L.waves_per_simd != 4)
+    throw Error("trajectory launch: waves_per_simd must be 0, 2 or 4");
   for (uint32_t j = 0; j < L.n_pairs; ++j) {
     const uint32_t qa = L.pairs[2 * j], qb = L.pairs[2 * j + 1];
     if (qa >= L.n_qubits || qb >= L.n_qubits || qa == qb)
@@ -446,10 +451,11 @@ void launch_traj(const cast_hip_traj_launch_t &L, hipStream_t stream) {
   a.log2_block = uint32_t(__builtin_ctz(L.block));
   for (uint32_t j = 0; j < 2 * L.n_pairs; ++j)
     a.pairs[j] = L.pairs[j];
+  const bool occ4 = L.waves_per_simd == 4;
   if (L.precision == 0)
-    launch_traj_t<float>(L, a, stream);
+    occ4 ? launch_traj_t<float, 4>(L, a, stream) : launch_traj_t<float, 2>(L, a, stream);
   else
-    launch_traj_t<double>(L, a, stream);
+    occ4 ? launch_traj_t<double, 4>(L, a, stream) : launch_traj_t<double, 2>(L, a, stream);
 }
 
 } // namespace cast_hip

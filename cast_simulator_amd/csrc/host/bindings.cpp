@@ -487,17 +487,19 @@ PYBIND11_MODULE(_host, m) {
 
   py::class_<TrajectoryTuning>(m, "TrajectoryTuning")
       .def(py::init([](uint32_t chunk_layers, uint32_t fusion_max, uint32_t block,
-                       uint32_t slots_per_cu, uint32_t max_slots) {
+                       uint32_t slots_per_cu, uint32_t max_slots, uint32_t waves_per_simd) {
              TrajectoryTuning t;
              t.chunk_layers = chunk_layers;
              t.fusion_max = fusion_max;
              t.block = block;
              t.slots_per_cu = slots_per_cu;
              t.max_slots = max_slots;
+             t.waves_per_simd = waves_per_simd;
              return t;
            }),
            py::arg("chunk_layers") = 4, py::arg("fusion_max") = 4, py::arg("block") = 256,
-           py::arg("slots_per_cu") = 2, py::arg("max_slots") = 0)
+           py::arg("slots_per_cu") = 2, py::arg("max_slots") = 0, py::arg("waves_per_simd") = 2)
+      .def_readwrite("waves_per_simd", &TrajectoryTuning::waves_per_simd)
       .def_readwrite("chunk_layers", &TrajectoryTuning::chunk_layers)
       .def_readwrite("fusion_max", &TrajectoryTuning::fusion_max)
       .def_readwrite("block", &TrajectoryTuning::block)
@@ -508,7 +510,8 @@ PYBIND11_MODULE(_host, m) {
                ", fusion_max=" + std::to_string(t.fusion_max) +
                ", block=" + std::to_string(t.block) +
                ", slots_per_cu=" + std::to_string(t.slots_per_cu) +
-               ", max_slots=" + std::to_string(t.max_slots) + ")";
+               ", max_slots=" + std::to_string(t.max_slots) +
+               ", waves_per_simd=" + std::to_string(t.waves_per_simd) + ")";
       });
 
   py::class_<TrajectoryRun>(m, "TrajectoryRun")
@@ -534,6 +537,7 @@ PYBIND11_MODULE(_host, m) {
       .def_readonly("block", &TrajectoryRun::block)
       .def_readonly("chunk_layers", &TrajectoryRun::chunk_layers)
       .def_readonly("fusion_max", &TrajectoryRun::fusion_max)
+      .def_readonly("waves_per_simd", &TrajectoryRun::waves_per_simd)
       .def("dump", [](const TrajectoryRun &r) {
         return py::array_t<double>(py::ssize_t(r.dump.size()), r.dump.data());
       });

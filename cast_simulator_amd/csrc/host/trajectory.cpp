@@ -431,6 +431,7 @@ TrajectoryRun TrajectoryEngine::run(const std::vector<TrajectorySample> &samples
     throw Error("not enough device memory for one trajectory slot");
   R.n_slots = uint32_t(n_slots);
   R.block = tn.block;
+  R.waves_per_simd = tn.waves_per_simd;
 
   slots_.ensure(n_slots * svb);
   pool_.ensure(std::max<size_t>(16, pool.size() * elem));
@@ -484,6 +485,7 @@ TrajectoryRun TrajectoryEngine::run(const std::vector<TrajectorySample> &samples
   L.out = out_.ptr;
   L.dump = dump ? dump_.ptr : 0;
   L.work = work_.ptr;
+  L.waves_per_simd = tn.waves_per_simd;
 
   void *e0 = cast_hip_event_create(err, kErr);
   if (!e0)
@@ -575,16 +577,24 @@ TrajectoryTuning TrajectoryEngine::autotune(const std::vector<TrajectorySample> 
     }
   };
   trial(best);
-  for (uint32_t block : {128u, 256u, 512u}) {
-    for (uint32_t spc : {1u, 2u, 3u, 4u}) {
-      TrajectoryTuning t = best;
-      t.block = block;
-      t.slots_per_cu = spc;
-      if (t.block == best.block && t.slots_per_cu == best.slots_per_cu)
-        continue;
-      trial(t);
-    }
-  }
+  // Launch shape: the resident workgroups per CU are bounded by the
+  // occupancy the kernel is built for, so only shapes that can be resident
+  // are tried (slots_per_cu x block <= 512 x waves_per_simd / 2).
+  const TrajectoryTuning start_shape = best;
+  for (uint32_t occ : {2u, 4u})
+    for (uint32_t block : {128u, 256u, 512u})
+      for (uint32_t spc : {1u, 2u, 4u}) {
+        if (spc * block > 256u * occ)
+          continue;
+        TrajectoryTuning t = start_shape;
+        t.waves_per_simd = occ;
+        t.block = block;
+        t.slots_per_cu = spc;
+        if (t.waves_per_simd == start_shape.waves_per_simd && t.block == start_shape.block &&
+            t.slots_per_cu == start_shape.slots_per_cu)
+          continue;
+        trial(t);
+      }
   const TrajectoryTuning shape = best;
   for (uint32_t kmax : {3u, 4u})
     for (uint32_t B : {1u, 2u, 4u, 8u}) {

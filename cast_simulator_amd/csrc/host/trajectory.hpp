@@ -77,6 +77,7 @@ struct TrajectoryTuning {
   uint32_t block = 256;      // threads per workgroup: 64, 128, 256 or 512
   uint32_t slots_per_cu = 2; // resident statevector slots per compute unit
   uint32_t max_slots = 0;    // 0 = no cap beyond memory and slots_per_cu
+  uint32_t waves_per_simd = 2; // kernel build: 2 (no spills) or 4 (two workgroups per CU)
 };
 
 struct TrajectoryRun {
@@ -91,7 +92,7 @@ struct TrajectoryRun {
   double min_norm_sq = 0.0, max_norm_sq = 0.0;
   double plan_s = 0.0, checkpoint_s = 0.0, gpu_s = 0.0, total_s = 0.0;
   uint64_t n_tasks = 0, n_gate_passes = 0, n_shared_clean = 0, n_dirty_matrices = 0;
-  uint32_t n_slots = 0, block = 0, chunk_layers = 0, fusion_max = 0;
+  uint32_t n_slots = 0, block = 0, chunk_layers = 0, fusion_max = 0, waves_per_simd = 0;
   std::vector<double> dump; // final states per task (interleaved f64), if asked for
 };
 

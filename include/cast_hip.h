@@ -310,6 +310,10 @@ typedef struct cast_hip_traj_launch_t {
   uint64_t dump;          /* 0, or device buffer for final states:      */
                           /* n_tasks * 2^n amplitudes, indexed by task  */
   uint64_t work;          /* device scratch: 2 * uint32_t (counter, error flag) */
+  uint32_t waves_per_simd;/* occupancy the kernel is built for: 2 (256 VGPRs, */
+                          /* no spills) or 4 (128 VGPRs, two 512-thread     */
+                          /* workgroups per CU); 0 = 2                      */
+  uint32_t reserved;      /* 0                                            */
 } cast_hip_traj_launch_t;
 
 /* Enqueue the persistent launch on `stream` (returns without waiting).  A

@@ -77,17 +77,28 @@ def test_tasks_match_oracle(n, prec):
     assert r.n_dirty_matrices > 0
 
 
+@pytest.mark.parametrize("waves", [2, 4])
 @pytest.mark.parametrize("block", [64, 128, 256, 512])
 @pytest.mark.parametrize("max_slots", [1, 3, 0])
-def test_launch_shapes(block, max_slots):
+def test_launch_shapes(block, max_slots, waves):
     n, D = 12, 6
     lc = cs.LayeredCircuit.random_layers(n, D, 5)
     S = cs.sample_trajectories(lc, cs.NoiseSpec(0.03, 1, D, block + max_slots), 20)
     pairs = [(0, 11), (3, 4), (10, 11), (5, 9)]
-    r = check_run(lc, S, F32, cs.TrajectoryTuning(2, 3, block, 2, max_slots), pairs)
+    r = check_run(lc, S, F32, cs.TrajectoryTuning(2, 3, block, 2, max_slots, waves), pairs)
     if max_slots:
         assert r.n_slots == min(max_slots, r.n_tasks)
-    assert r.block == block
+    assert r.block == block and r.waves_per_simd == waves
+
+
+@pytest.mark.parametrize("prec", [F64, F32])
+def test_high_occupancy_build_matches_oracle(prec):
+    n, D = 13, 8
+    lc = cs.LayeredCircuit.random_layers(n, D, 44)
+    S = cs.sample_trajectories(lc, cs.NoiseSpec(0.05, 1, D, 45), 24)
+    init = np.asarray(cs.random_statevector(n, 46))
+    pairs = [(i, i + 1) for i in range(n - 1)]
+    check_run(lc, S, prec, cs.TrajectoryTuning(3, 4, 512, 2, 0, 4), pairs, init)
 
 
 @pytest.mark.parametrize("B,kmax", [(1, 2), (4, 4), (6, 3)])
@@ -210,6 +221,10 @@ def test_abi_rejects_bad_launch_shapes():
     with pytest.raises(abi.AbiError, match="pair 0"):
         abi.traj_run(L)
     L.pairs[1] = 3
+    L.waves_per_simd = 3
+    with pytest.raises(abi.AbiError, match="waves_per_simd"):
+        abi.traj_run(L)
+    L.waves_per_simd = 4
     L.precision = 2
     with pytest.raises(abi.AbiError, match="precision"):
         abi.traj_run(L)
