@@ -32,6 +32,7 @@ from ._host import (  # noqa: E402,F401
     CastError,
     CircuitGraph,
     CostModel,
+    ExploredBranch,
     FusionConfig,
     FusionDecision,
     FusionLog,
@@ -46,6 +47,8 @@ from ._host import (  # noqa: E402,F401
     ProfileConfig,
     QuantumCircuit,
     QuantumGate,
+    QuantumState,
+    Representation,
     Simulator,
     ServeConfig,
     ServeRequest,
@@ -53,11 +56,14 @@ from ._host import (  # noqa: E402,F401
     Server,
     SizeOnlyCostModel,
     TrajectoryEngine,
+    TrajectoryOpts,
+    TrajectoryResult,
     TrajectoryRun,
     TrajectorySample,
     TrajectoryTuning,
     adaptive_sweep,
     apply_size_two_fusion,
+    batch_sample,
     compute_bw_slope,
     expand_matrix,
     fit_from_samples,

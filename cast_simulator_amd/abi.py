@@ -165,6 +165,10 @@ _SIGS = {
     "cast_hip_traj_run": (_i, [C.POINTER(TrajLaunch), _vp, _err, _sz]),
     "cast_hip_inner_product": (
         _i, [_u64, _u8, _u64, _u8, _u32, C.POINTER(C.c_double), _vp, _err, _sz]),
+    "cast_hip_marginal_probabilities": (
+        _i, [_u64, _u32, _u8, C.POINTER(_u32), _sz, C.POINTER(C.c_double), _vp, _err, _sz]),
+    "cast_hip_dm_populations": (
+        _i, [_u64, _u32, _u8, C.POINTER(C.c_double), _vp, _err, _sz]),
 }
 
 
@@ -390,6 +394,23 @@ def inner_product(a: int, prec_a: int, b: int, prec_b: int, n_qubits: int,
     out = (C.c_double * 2)()
     _call("cast_hip_inner_product", a, prec_a, b, prec_b, n_qubits, out, stream)
     return complex(out[0], out[1])
+
+
+def marginal_probabilities(sv: int, n_qubits: int, precision: int, measured,
+                           stream: int | None = None) -> np.ndarray:
+    out = np.zeros(1 << len(measured), dtype=np.float64)
+    q = (_u32 * max(1, len(measured)))(*measured)
+    _call("cast_hip_marginal_probabilities", sv, n_qubits, precision, q, len(measured),
+          out.ctypes.data_as(C.POINTER(C.c_double)), stream)
+    return out
+
+
+def dm_populations(sv: int, n_physical: int, precision: int,
+                   stream: int | None = None) -> np.ndarray:
+    out = np.zeros(1 << n_physical, dtype=np.float64)
+    _call("cast_hip_dm_populations", sv, n_physical, precision,
+          out.ctypes.data_as(C.POINTER(C.c_double)), stream)
+    return out
 
 
 def traj_run(args: TrajLaunch, stream: int | None = None) -> None:

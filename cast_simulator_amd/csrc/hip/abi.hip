@@ -727,4 +727,36 @@ int cast_hip_inner_product(uint64_t a, uint8_t precision_a, uint64_t b, uint8_t 
   });
 }
 
+int cast_hip_marginal_probabilities(uint64_t sv_dptr, uint32_t sv_n_qubits, uint8_t precision,
+                                    const uint32_t *measured_qubits, size_t n_measured,
+                                    double *out, void *stream, char *err, size_t len) {
+  return guard(err, len, [&] {
+    check_precision(precision);
+    if (!sv_dptr || !out)
+      throw Error("null statevector or output pointer");
+    if (sv_n_qubits < 1 || sv_n_qubits > 40)
+      throw Error("statevector qubit count out of range");
+    if (n_measured && !measured_qubits)
+      throw Error("null measured-qubit list");
+    if (n_measured > sv_n_qubits)
+      throw Error("more measured qubits than the statevector has");
+    marginal_probabilities(reinterpret_cast<const void *>(sv_dptr), sv_n_qubits, precision,
+                           measured_qubits, uint32_t(n_measured), out,
+                           static_cast<hipStream_t>(stream));
+  });
+}
+
+int cast_hip_dm_populations(uint64_t sv_dptr, uint32_t n_physical, uint8_t precision, double *out,
+                            void *stream, char *err, size_t len) {
+  return guard(err, len, [&] {
+    check_precision(precision);
+    if (!sv_dptr || !out)
+      throw Error("null statevector or output pointer");
+    if (n_physical < 1 || n_physical > 20)
+      throw Error("density-matrix qubit count out of range (1..20)");
+    dm_populations(reinterpret_cast<const void *>(sv_dptr), n_physical, precision, out,
+                   static_cast<hipStream_t>(stream));
+  });
+}
+
 } // extern "C"

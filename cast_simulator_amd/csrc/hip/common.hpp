@@ -119,6 +119,15 @@ void expect_zz(const void *sv, uint32_t n_qubits, uint8_t precision, const uint3
 void inner_product(const void *a, uint8_t prec_a, const void *b, uint8_t prec_b,
                    uint32_t n_qubits, double *out, hipStream_t stream);
 
+// Marginal probabilities over `m` measured qubits (bin bit i = address bit
+// qubits[i]) and the density-matrix diagonal, f64, into host memory;
+// synchronous on `stream` (sv_kernels.hip).
+void marginal_probabilities(const void *sv, uint32_t n_qubits, uint8_t precision,
+                            const uint32_t *qubits, uint32_t m, double *out_host,
+                            hipStream_t stream);
+void dm_populations(const void *sv, uint32_t n_physical, uint8_t precision, double *out_host,
+                    hipStream_t stream);
+
 // Deterministic f64 norm + pair sums into device `out` (sv_kernels.hip).
 size_t observe_scratch_bytes(uint32_t n_pairs);
 void launch_observe(const void *sv, uint32_t n_qubits, uint8_t precision, const uint32_t *pairs,

@@ -10,7 +10,9 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <memory>
 #include <mutex>
+#include <string>
 #include <type_traits>
 #include <vector>
 
@@ -193,6 +195,92 @@ __global__ void __launch_bounds__(kRedBlock) k_inner(const TA *__restrict__ a,
   }
 }
 
+// Marginal probabilities over measured qubits pos[0..m): bin b collects
+// |amp_j|^2 for every j whose bit pos[i] equals bit i of b (the reference's
+// compress_bits, src/types/gate.rs:636-643, summed as in
+// src/simulator/measure.rs:15-25).  Amplitude (b, r) of bin b and rest index
+// r is j = scatter(b) | pdep(r, ~M), M the measured mask.
+constexpr unsigned kMaxMarginalQubits = 40;
+struct MarginalArgs {
+  uint64_t mask;                  // M
+  uint32_t m;                     // measured qubits
+  uint32_t pos[kMaxMarginalQubits];
+};
+
+__device__ __forceinline__ uint64_t insert_zero_u64(uint64_t x, uint32_t p) {
+  const uint64_t lo = (uint64_t(1) << p) - 1;
+  return ((x & ~lo) << 1) | (x & lo);
+}
+
+__device__ __forceinline__ uint64_t scatter_bin(uint64_t b, const MarginalArgs &a) {
+  uint64_t base = 0;
+#pragma unroll
+  for (unsigned i = 0; i < kMaxMarginalQubits; ++i)
+    if (i < a.m)
+      base |= ((b >> i) & 1u) << a.pos[i];
+  return base;
+}
+
+__device__ __forceinline__ uint64_t deposit_rest(uint64_t r, uint64_t mask) {
+  for (uint64_t mm = mask; mm; mm &= mm - 1) // ascending measured positions
+    r = insert_zero_u64(r, uint32_t(__builtin_ctzll(mm)));
+  return r;
+}
+
+template <typename T> __device__ __forceinline__ double amp_prob(const T *sv, uint64_t j) {
+  using VT = typename std::conditional<sizeof(T) == 8, double2, float2>::type;
+  const VT v = reinterpret_cast<const VT *>(sv)[j];
+  const double re = double(v.x), im = double(v.y);
+  return __builtin_fma(re, re, im * im);
+}
+
+// Rest-major: workgroup w = c * B + b sums chunk c (chunk_len rest indices,
+// consecutive across the workgroup's threads) of bin b into partial[w].
+template <typename T>
+__global__ void __launch_bounds__(kRedBlock) k_marginal_rest(const T *__restrict__ sv,
+                                                             MarginalArgs a, uint64_t chunk_len,
+                                                             double *__restrict__ partial) {
+  __shared__ double lds[kRedBlock / 64];
+  const uint64_t w = uint64_t(blockIdx.y) * gridDim.x + blockIdx.x;
+  const uint64_t b = w & ((uint64_t(1) << a.m) - 1), c = w >> a.m;
+  const uint64_t base = scatter_bin(b, a);
+  double acc = 0.0;
+  const uint64_t r0 = c * chunk_len;
+  for (uint64_t r = r0 + threadIdx.x; r < r0 + chunk_len; r += blockDim.x)
+    acc += amp_prob(sv, deposit_rest(r, a.mask) | base);
+  const double t = block_sum(acc, lds);
+  if (threadIdx.x == 0)
+    partial[w] = t;
+}
+
+// Bin-major (few amplitudes per bin): one thread per bin, rest indices in
+// ascending order.
+template <typename T>
+__global__ void __launch_bounds__(kRedBlock) k_marginal_bin(const T *__restrict__ sv,
+                                                            MarginalArgs a, uint64_t n_rest,
+                                                            double *__restrict__ out) {
+  const uint64_t n_bins = uint64_t(1) << a.m;
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t b = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; b < n_bins; b += stride) {
+    const uint64_t base = scatter_bin(b, a);
+    double acc = 0.0;
+    for (uint64_t r = 0; r < n_rest; ++r)
+      acc += amp_prob(sv, deposit_rest(r, a.mask) | base);
+    out[b] = acc;
+  }
+}
+
+// Density-matrix diagonal: the 2n-qubit vectorised rho holds rho[i][i] at
+// index i | (i << n) (src/simulator/mod.rs:349-363); out[i] = Re of it.
+template <typename T>
+__global__ void __launch_bounds__(kRedBlock) k_dm_diag(const T *__restrict__ sv, uint32_t n,
+                                                       double *__restrict__ out) {
+  const uint64_t dim = uint64_t(1) << n;
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < dim; i += stride)
+    out[i] = double(sv[2 * (i | (i << n))]);
+}
+
 unsigned red_grid(uint64_t work) {
   const uint64_t need = (work + kRedBlock - 1) / kRedBlock;
   const uint64_t cap = std::min<uint64_t>(kMaxRedGrid, uint64_t(device_props().n_cu) * 4);
@@ -357,6 +445,119 @@ void inner_product(const void *a, uint8_t prec_a, const void *b, uint8_t prec_b,
   check(hipStreamSynchronize(stream), "hipStreamSynchronize(inner)");
   out[0] = s.h_out[0];
   out[1] = s.h_out[1];
+}
+
+} // namespace cast_hip
+
+namespace cast_hip {
+
+namespace {
+// Device buffer for one synchronous call: small sizes from the block cache,
+// large ones straight from the allocator.
+struct CallBuffer {
+  void *p = nullptr;
+  size_t bytes = 0;
+  explicit CallBuffer(size_t b) : bytes(b) {
+    if (bytes <= (size_t(16) << 20))
+      p = cache_alloc(bytes);
+    else
+      check(hipMalloc(&p, bytes), "hipMalloc(marginal buffer)");
+  }
+  ~CallBuffer() {
+    if (!p)
+      return;
+    if (bytes <= (size_t(16) << 20))
+      cache_release(p, bytes);
+    else
+      (void)hipFree(p);
+  }
+  CallBuffer(const CallBuffer &) = delete;
+  CallBuffer &operator=(const CallBuffer &) = delete;
+};
+
+dim3 wide_grid(uint64_t blocks) {
+  const uint64_t cap = 1u << 20;
+  if (blocks <= cap)
+    return dim3(unsigned(blocks));
+  return dim3(unsigned(cap), unsigned(blocks / cap));
+}
+} // namespace
+
+void marginal_probabilities(const void *sv, uint32_t n_qubits, uint8_t precision,
+                            const uint32_t *qubits, uint32_t m, double *out_host,
+                            hipStream_t stream) {
+  if (m > kMaxMarginalQubits || m > n_qubits)
+    throw Error("marginal: more measured qubits than the statevector has");
+  MarginalArgs a{};
+  a.m = m;
+  for (uint32_t i = 0; i < m; ++i) {
+    if (qubits[i] >= n_qubits)
+      throw Error("marginal: measured qubit " + std::to_string(qubits[i]) + " out of range");
+    if (a.mask >> qubits[i] & 1u)
+      throw Error("marginal: measured qubit " + std::to_string(qubits[i]) + " repeated");
+    a.mask |= uint64_t(1) << qubits[i];
+    a.pos[i] = qubits[i];
+  }
+  const uint64_t n_bins = uint64_t(1) << m, n_rest = uint64_t(1) << (n_qubits - m);
+  CallBuffer out(sizeof(double) * n_bins);
+  double *d_out = static_cast<double *>(out.p);
+  if (n_rest >= 1024) {
+    const uint64_t chunk = std::min<uint64_t>(
+        n_rest, std::max<uint64_t>(1024, (uint64_t(1) << n_qubits) >> 14));
+    const uint64_t n_chunks = n_rest / chunk, blocks = n_bins * n_chunks;
+    std::unique_ptr<CallBuffer> part;
+    double *d_part = d_out;
+    if (n_chunks > 1) {
+      part = std::make_unique<CallBuffer>(sizeof(double) * blocks);
+      d_part = static_cast<double *>(part->p);
+    }
+    if (precision == 0)
+      hipLaunchKernelGGL(k_marginal_rest<float>, wide_grid(blocks), dim3(kRedBlock), 0, stream,
+                         static_cast<const float *>(sv), a, chunk, d_part);
+    else
+      hipLaunchKernelGGL(k_marginal_rest<double>, wide_grid(blocks), dim3(kRedBlock), 0, stream,
+                         static_cast<const double *>(sv), a, chunk, d_part);
+    check(hipGetLastError(), "hipLaunchKernelGGL(k_marginal_rest)");
+    if (n_chunks > 1) {
+      hipLaunchKernelGGL(k_sum_partials, dim3(unsigned(n_bins)), dim3(kRedBlock), 0, stream,
+                         d_part, unsigned(n_chunks), unsigned(n_bins), d_out);
+      check(hipGetLastError(), "hipLaunchKernelGGL(k_sum_partials)");
+    }
+    check(hipMemcpyAsync(out_host, d_out, sizeof(double) * n_bins, hipMemcpyDeviceToHost, stream),
+          "hipMemcpyAsync(marginal)");
+    check(hipStreamSynchronize(stream), "hipStreamSynchronize(marginal)");
+    return;
+  }
+  const unsigned grid = unsigned(std::min<uint64_t>((n_bins + kRedBlock - 1) / kRedBlock,
+                                                    uint64_t(device_props().n_cu) * 16));
+  if (precision == 0)
+    hipLaunchKernelGGL(k_marginal_bin<float>, dim3(grid), dim3(kRedBlock), 0, stream,
+                       static_cast<const float *>(sv), a, n_rest, d_out);
+  else
+    hipLaunchKernelGGL(k_marginal_bin<double>, dim3(grid), dim3(kRedBlock), 0, stream,
+                       static_cast<const double *>(sv), a, n_rest, d_out);
+  check(hipGetLastError(), "hipLaunchKernelGGL(k_marginal_bin)");
+  check(hipMemcpyAsync(out_host, d_out, sizeof(double) * n_bins, hipMemcpyDeviceToHost, stream),
+        "hipMemcpyAsync(marginal)");
+  check(hipStreamSynchronize(stream), "hipStreamSynchronize(marginal)");
+}
+
+void dm_populations(const void *sv, uint32_t n_physical, uint8_t precision, double *out_host,
+                    hipStream_t stream) {
+  const uint64_t dim = uint64_t(1) << n_physical;
+  CallBuffer out(sizeof(double) * dim);
+  const unsigned grid = unsigned(std::min<uint64_t>((dim + kRedBlock - 1) / kRedBlock,
+                                                    uint64_t(device_props().n_cu) * 16));
+  if (precision == 0)
+    hipLaunchKernelGGL(k_dm_diag<float>, dim3(grid), dim3(kRedBlock), 0, stream,
+                       static_cast<const float *>(sv), n_physical, static_cast<double *>(out.p));
+  else
+    hipLaunchKernelGGL(k_dm_diag<double>, dim3(grid), dim3(kRedBlock), 0, stream,
+                       static_cast<const double *>(sv), n_physical, static_cast<double *>(out.p));
+  check(hipGetLastError(), "hipLaunchKernelGGL(k_dm_diag)");
+  check(hipMemcpyAsync(out_host, out.p, sizeof(double) * dim, hipMemcpyDeviceToHost, stream),
+        "hipMemcpyAsync(dm populations)");
+  check(hipStreamSynchronize(stream), "hipStreamSynchronize(dm populations)");
 }
 
 } // namespace cast_hip

@@ -8,6 +8,8 @@
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
+#include <optional>
+
 #include "circuit.hpp"
 #include "cost_model.hpp"
 #include "fusion.hpp"
@@ -335,7 +337,61 @@ PYBIND11_MODULE(_host, m) {
       .def("clone_device", &HipStatevector::clone_device)
       .def("inner", &HipStatevector::inner)
       .def("fidelity", &HipStatevector::fidelity)
-      .def("expect_zz", &HipStatevector::expect_zz);
+      .def("expect_zz", &HipStatevector::expect_zz)
+      .def("marginal_probabilities",
+           [](const HipStatevector &s, const std::vector<uint32_t> &measured) {
+             return vec_to_numpy(marginal_probabilities(s, measured));
+           });
+
+  py::enum_<Representation>(m, "Representation")
+      .value("StateVector", Representation::StateVector)
+      .value("DensityMatrix", Representation::DensityMatrix);
+
+  py::class_<QuantumState>(m, "QuantumState")
+      .def("n_qubits", &QuantumState::n_qubits)
+      .def("is_pure", &QuantumState::is_pure)
+      .def("representation", &QuantumState::representation)
+      .def("populations", [](const QuantumState &s) { return vec_to_numpy(s.populations()); })
+      .def("trace", &QuantumState::trace)
+      .def("download_amplitudes",
+           [](const QuantumState &s) { return vec_to_numpy(s.download_amplitudes()); })
+      .def("statevector", &QuantumState::statevector, py::return_value_policy::reference_internal);
+
+  py::class_<TrajectoryOpts>(m, "TrajectoryOpts")
+      .def(py::init([](std::vector<uint32_t> measured, uint64_t n_samples,
+                       std::optional<uint64_t> seed, std::optional<size_t> max_ensemble) {
+             TrajectoryOpts o;
+             o.measured_qubits = std::move(measured);
+             o.n_samples = n_samples;
+             o.has_seed = seed.has_value();
+             o.seed = seed.value_or(0);
+             o.max_ensemble = max_ensemble.value_or(1);
+             return o;
+           }),
+           py::arg("measured_qubits"), py::arg("n_samples"), py::arg("seed") = py::none(),
+           py::arg("max_ensemble") = py::none())
+      .def_readwrite("measured_qubits", &TrajectoryOpts::measured_qubits)
+      .def_readwrite("n_samples", &TrajectoryOpts::n_samples)
+      .def_readwrite("max_ensemble", &TrajectoryOpts::max_ensemble);
+
+  py::class_<ExploredBranch>(m, "ExploredBranch")
+      .def_readonly("weight", &ExploredBranch::weight)
+      .def_readonly("noise_path", &ExploredBranch::noise_path)
+      .def_readonly("n_samples", &ExploredBranch::n_samples);
+
+  py::class_<TrajectoryResult>(m, "TrajectoryResult")
+      .def_readonly("histogram", &TrajectoryResult::histogram)
+      .def_readonly("n_samples", &TrajectoryResult::n_samples)
+      .def_readonly("branches", &TrajectoryResult::branches)
+      .def_readonly("explored_weight", &TrajectoryResult::explored_weight);
+
+  m.def(
+      "batch_sample",
+      [](const std::vector<double> &probs, uint64_t n_samples, uint64_t seed) {
+        Rng rng(seed);
+        return batch_sample(probs, n_samples, rng);
+      },
+      py::arg("probs"), py::arg("n_samples"), py::arg("seed"));
 
   py::class_<PlanInfo>(m, "PlanInfo")
       .def_readonly("k", &PlanInfo::k)
@@ -399,7 +455,14 @@ PYBIND11_MODULE(_host, m) {
                ks.append(py::make_tuple(k.id, k.gpu_s));
              return py::make_tuple(ks, st.wall_s);
            })
-      .def("simulate", &Simulator::simulate)
+      .def("simulate", py::overload_cast<const CircuitGraph &>(&Simulator::simulate))
+      .def("simulate",
+           py::overload_cast<const CircuitGraph &, Representation>(&Simulator::simulate))
+      .def("sample_trajectory",
+           [](Simulator &s, const CircuitGraph &g, const TrajectoryOpts &o) {
+             py::gil_scoped_release rel;
+             return s.sample_trajectory(g, o);
+           })
       .def("bench",
            [](Simulator &s, const CircuitGraph &g, double budget) {
              const auto r = s.bench(g, budget);

@@ -175,6 +175,65 @@ private:
   std::map<std::pair<KernelId, uint32_t>, TuneState> tune_;
 };
 
+// Marginal measurement probabilities over `measured` (bin bit i = qubit
+// measured[i]), computed on the device in f64 (cast_hip_marginal_probabilities;
+// the reference downloads and accumulates, src/simulator/mod.rs:282-291).
+std::vector<double> marginal_probabilities(const HipStatevector &sv,
+                                           const std::vector<uint32_t> &measured);
+
+// src/simulator/mod.rs:418-432.
+enum class Representation { StateVector, DensityMatrix };
+
+// Final state of Simulator::simulate (src/simulator/mod.rs:320-414): a pure
+// state on n qubits or a density matrix held as a 2n-qubit vectorised state
+// (rho[i][j] at index i | (j << n)).
+class QuantumState {
+public:
+  QuantumState(HipStatevector sv, uint32_t n_physical, Representation repr)
+      : sv_(std::move(sv)), n_(n_physical), repr_(repr) {}
+  uint32_t n_qubits() const { return n_; }
+  Representation representation() const { return repr_; }
+  bool is_pure() const { return repr_ == Representation::StateVector; }
+  const HipStatevector &statevector() const { return sv_; }
+  // |a_i|^2 (pure) or Re rho[i][i] (density matrix), from the device.
+  std::vector<double> populations() const;
+  // Squared norm (pure) or Tr rho.
+  double trace() const;
+  std::vector<double> download_amplitudes() const { return sv_.download(); }
+
+private:
+  HipStatevector sv_;
+  uint32_t n_;
+  Representation repr_;
+};
+
+// Ensemble trajectory sampling (src/simulator/trajectory.rs:17-60).
+struct TrajectoryOpts {
+  std::vector<uint32_t> measured_qubits;
+  uint64_t n_samples = 0;
+  bool has_seed = false;   // false: seeded from std::random_device
+  uint64_t seed = 0;
+  size_t max_ensemble = 1; // 0 is treated as 1
+};
+
+struct ExploredBranch {
+  double weight = 0.0;
+  std::vector<size_t> noise_path;
+  uint64_t n_samples = 0;
+};
+
+struct TrajectoryResult {
+  std::map<uint64_t, uint64_t> histogram;
+  uint64_t n_samples = 0;
+  std::vector<ExploredBranch> branches; // ensemble order: weight descending
+  double explored_weight = 0.0;
+};
+
+// Inverse-CDF batch sampling (src/simulator/measure.rs:45-78): outcome =
+// first index whose running sum exceeds u ~ U[0, 1), clamped to the last.
+std::map<uint64_t, uint64_t> batch_sample(const std::vector<double> &probs, uint64_t n_samples,
+                                          Rng &rng);
+
 // Simulator<Hip>: state-vector simulation of a circuit graph.
 class Simulator {
 public:
@@ -192,6 +251,13 @@ public:
   SyncStats run(const CircuitGraph &graph, HipStatevector &sv);
   // New |0...0> statevector of graph.n_qubits(), simulated.
   HipStatevector simulate(const CircuitGraph &graph);
+  // src/simulator/mod.rs:523-547 with prepare_gates (:586-618): for
+  // DensityMatrix every gate is lifted to its superoperator on 2n qubits.
+  QuantumState simulate(const CircuitGraph &graph, Representation repr);
+
+  // Deterministic top-M ensemble over noise branches, then measurement
+  // samples from each surviving member (src/simulator/trajectory.rs:95-312).
+  TrajectoryResult sample_trajectory(const CircuitGraph &graph, const TrajectoryOpts &opts);
 
   struct BenchResult {
     double compile_s = 0.0;

@@ -253,6 +253,25 @@ int cast_hip_observe_async(uint64_t sv_dptr, uint32_t sv_n_qubits, uint8_t preci
                            const uint32_t *pairs, size_t n_pairs, uint64_t out, uint64_t scratch,
                            void *stream, char *err_buf, size_t err_buf_len);
 
+/* Marginal measurement probabilities over `measured_qubits` (distinct, any
+ * order): out[b] = sum of |amp_j|^2 over every j whose bit measured_qubits[i]
+ * equals bit i of b, for b < 2^n_measured, accumulated in f64 on the device;
+ * only the 2^n_measured results cross to the host (`out`, host memory).
+ * Synchronous on `stream`.  Replaces the download-and-accumulate of
+ * Backend::marginal_probabilities for Cuda (src/simulator/mod.rs:282-291,
+ * src/simulator/measure.rs:15-25). */
+int cast_hip_marginal_probabilities(uint64_t sv_dptr, uint32_t sv_n_qubits, uint8_t precision,
+                                    const uint32_t *measured_qubits, size_t n_measured,
+                                    double *out, void *stream, char *err_buf, size_t err_buf_len);
+
+/* Populations of a density matrix held as a 2 n_physical-qubit vectorised
+ * statevector: out[i] = Re rho[i][i], the element at index i | (i << n),
+ * for i < 2^n_physical (host memory), f64.  Synchronous on `stream`.
+ * Replaces the download in QuantumState<Cuda>::populations / trace
+ * (src/simulator/mod.rs:388-414). */
+int cast_hip_dm_populations(uint64_t sv_dptr, uint32_t n_physical, uint8_t precision, double *out,
+                            void *stream, char *err_buf, size_t err_buf_len);
+
 /* <a|b> = sum_j conj(a_j) b_j over two statevectors of sv_n_qubits qubits,
  * each F32 or F64, accumulated in f64; out_re_im = {Re, Im}.  Synchronous on
  * `stream`.  Fidelity |<a|b>|^2 / (<a|a><b|b>) of a reduced-precision result

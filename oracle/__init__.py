@@ -6,8 +6,9 @@ follows).  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
 leg import this module, always as the checker or the timed CPU baseline,
 never as part of the product path.
 
-Every function here is pinned by tests/test_oracle.py; none is
-"parity unpinned".
+Every function here is pinned by tests/test_oracle.py or (marginal,
+dm_populations, simulate_dm, ensemble, samples_per_member)
+tests/test_ensemble_host.py; none is "parity unpinned".
 """
 from __future__ import annotations
 
@@ -65,6 +66,10 @@ def lib() -> C.CDLL:
             f.argtypes = [p, C.c_uint, C.c_uint, _dp, _u32p, C.c_uint, C.c_uint]
         L.orc_expect_zz_f64.restype = None
         L.orc_expect_zz_f64.argtypes = [_dp, C.c_uint, _u32p, C.c_size_t, _dp]
+        L.orc_marginal_f64.restype = None
+        L.orc_marginal_f64.argtypes = [_dp, C.c_uint, _u32p, C.c_uint, _dp]
+        L.orc_dm_populations_f64.restype = None
+        L.orc_dm_populations_f64.argtypes = [_dp, C.c_uint, _dp]
         L.orc_simd_len.restype = C.c_size_t
         L.orc_simd_len.argtypes = [C.c_uint, C.c_uint]
         L.orc_num_cpus.restype = C.c_uint
@@ -129,6 +134,78 @@ def expect_zz(sv: np.ndarray, pairs) -> np.ndarray:
     out = np.zeros(len(pairs), dtype=np.float64)
     lib().orc_expect_zz_f64(x.ctypes.data_as(_dp), n, (C.c_uint32 * len(flat))(*flat),
                             len(pairs), out.ctypes.data_as(_dp))
+    return out
+
+
+def marginal(sv: np.ndarray, measured) -> np.ndarray:
+    """Marginal probabilities over `measured` (bin bit i = qubit measured[i])."""
+    x = np.ascontiguousarray(sv, dtype=np.float64)
+    n = int(x.size // 2).bit_length() - 1
+    out = np.zeros(1 << len(measured), dtype=np.float64)
+    lib().orc_marginal_f64(x.ctypes.data_as(_dp), n, _q(list(measured)) if measured else None,
+                           len(measured), out.ctypes.data_as(_dp))
+    return out
+
+
+def dm_populations(rho: np.ndarray, n_physical: int) -> np.ndarray:
+    """Diagonal of a density matrix held as a 2n-qubit vectorised state."""
+    x = np.ascontiguousarray(rho, dtype=np.float64)
+    out = np.zeros(1 << n_physical, dtype=np.float64)
+    lib().orc_dm_populations_f64(x.ctypes.data_as(_dp), n_physical, out.ctypes.data_as(_dp))
+    return out
+
+
+def simulate_dm(n_physical: int, gates) -> np.ndarray:
+    """Density-matrix simulation (src/simulator/mod.rs:523-547 with
+    prepare_gates :607-615): rho = |0><0| as a 2n-qubit vector, each gate
+    given as its already-lifted (superoperator, qubits) pair."""
+    rho = np.zeros(2 << (2 * n_physical))
+    rho[0] = 1.0
+    for m, q in gates:
+        apply_gate(rho, m, q)
+    return rho
+
+
+def ensemble(n: int, gates, max_ensemble: int):
+    """Top-M ensemble over noise branches (src/simulator/trajectory.rs:186-265),
+    pure Python over the oracle's apply_gate; small cases only.
+
+    gates: list of ("u", matrix, qubits) or ("noise", [(p, kraus), ...], qubits).
+    Candidates are ordered by weight, descending, ties in (member, branch)
+    order (a stable sort; the reference's unstable sort leaves ties in an
+    unspecified order).  Returns [(weight, noise_path, statevector)]."""
+    sv = np.zeros(2 << n)
+    sv[0] = 1.0
+    ens = [(1.0, [], sv)]
+    M = max(1, max_ensemble)
+    for g in gates:
+        if g[0] == "u":
+            for _, _, x in ens:
+                apply_gate(x, g[1], g[2])
+            continue
+        branches, q = g[1], g[2]
+        cand = [(mi, bi, ens[mi][0] * p) for mi in range(len(ens))
+                for bi, (p, _) in enumerate(branches)]
+        cand.sort(key=lambda c: -c[2])  # stable
+        cand = cand[:M]
+        nxt = []
+        for mi, bi, w in cand:
+            x = ens[mi][2].copy()
+            apply_gate(x, branches[bi][1], q)
+            nxt.append((w, ens[mi][1] + [bi], x))
+        ens = nxt
+    return ens
+
+
+def samples_per_member(weights, n_samples: int):
+    """Samples drawn from each ensemble member (trajectory.rs:283-293)."""
+    total = sum(weights)
+    out, remaining = [], n_samples
+    for i, w in enumerate(weights):
+        s = remaining if i == len(weights) - 1 else min(int(np.floor(w / total * n_samples)),
+                                                         remaining)
+        remaining -= min(remaining, s)
+        out.append(s)
     return out
 
 

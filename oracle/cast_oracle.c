@@ -129,6 +129,24 @@ void orc_scale_f32(float *x, size_t n, double factor) {
     x[i] = (float)((double)x[i] * factor);
 }
 
+void orc_marginal_f64(const double *sv, unsigned n, const uint32_t *pos, unsigned m,
+                      double *out) {
+  const uint64_t N = (uint64_t)1 << n;
+  for (uint64_t b = 0; b < ((uint64_t)1 << m); ++b)
+    out[b] = 0.0;
+  for (uint64_t j = 0; j < N; ++j) {
+    uint64_t b = 0;
+    for (unsigned t = 0; t < m; ++t)
+      b |= ((j >> pos[t]) & 1u) << t;
+    out[b] += sv[2 * j] * sv[2 * j] + sv[2 * j + 1] * sv[2 * j + 1];
+  }
+}
+
+void orc_dm_populations_f64(const double *rho, unsigned n, double *out) {
+  for (uint64_t i = 0; i < ((uint64_t)1 << n); ++i)
+    out[i] = rho[2 * (i | (i << n))];
+}
+
 void orc_expect_zz_f64(const double *sv, unsigned n, const uint32_t *pairs, size_t n_pairs,
                        double *out) {
   const uint64_t N = (uint64_t)1 << n;

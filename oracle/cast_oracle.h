@@ -63,6 +63,17 @@ void orc_scale_f32(float *x, size_t n_elements, double factor);
 void orc_expect_zz_f64(const double *sv, unsigned n, const uint32_t *pairs, size_t n_pairs,
                        double *out);
 
+/* Marginal probabilities over measured qubits pos[0..m): out[b] (2^m entries,
+ * zeroed here) += |amp_j|^2 at b = compress_bits(j, pos), visiting j in
+ * ascending order (src/simulator/measure.rs:15-25, compress_bits
+ * src/types/gate.rs:636-643). */
+void orc_marginal_f64(const double *sv, unsigned n, const uint32_t *pos, unsigned m,
+                      double *out);
+
+/* Density-matrix populations: out[i] = Re of element i | (i << n) of the
+ * 2n-qubit vectorised rho, i < 2^n (src/simulator/mod.rs:351-366). */
+void orc_dm_populations_f64(const double *rho, unsigned n, double *out);
+
 /* ── SIMD split-layout baseline ──────────────────────────────────────────── */
 
 /* Scalars needed for an n-qubit statevector in split layout with 2^s lanes

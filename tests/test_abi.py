@@ -104,3 +104,24 @@ def test_destroy_functions_accept_null():
     L.cast_hip_stream_destroy(None)
     L.cast_hip_event_destroy(None)
     L.cast_hip_sv_free(0)
+
+
+def test_measurement_entry_points_validate_before_the_device():
+    buf = C.create_string_buffer(abi.ERR_LEN)
+    L = abi.lib()
+    out = (C.c_double * 4)()
+    q = (C.c_uint32 * 2)(0, 1)
+    assert L.cast_hip_marginal_probabilities(0, 8, 1, q, 2, out, None, buf, abi.ERR_LEN) == 1
+    assert b"null statevector" in buf.value
+    assert L.cast_hip_marginal_probabilities(64, 8, 3, q, 2, out, None, buf, abi.ERR_LEN) == 1
+    assert b"precision" in buf.value
+    assert L.cast_hip_marginal_probabilities(64, 41, 1, q, 2, out, None, buf, abi.ERR_LEN) == 1
+    assert b"out of range" in buf.value
+    assert L.cast_hip_marginal_probabilities(64, 1, 1, q, 2, out, None, buf, abi.ERR_LEN) == 1
+    assert b"more measured qubits" in buf.value
+    assert L.cast_hip_marginal_probabilities(64, 8, 1, None, 2, out, None, buf, abi.ERR_LEN) == 1
+    assert b"null measured" in buf.value
+    assert L.cast_hip_dm_populations(64, 21, 1, out, None, buf, abi.ERR_LEN) == 1
+    assert b"1..20" in buf.value
+    assert L.cast_hip_dm_populations(64, 4, 1, None, None, buf, abi.ERR_LEN) == 1
+    assert b"null" in buf.value
