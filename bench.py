@@ -229,6 +229,76 @@ def run_dense_large(cs, n64: int = 33, n32: int = 34, depth: int = 12):
     }
 
 
+def run_marginals(cs, n: int = 30, reps: int = 3):
+    """Device marginal probabilities on an n-qubit f64 statevector: one read of
+    every amplitude (2^n * 16 bytes) per call; only 2^m doubles reach the
+    host.  The reference downloads the whole statevector instead
+    (src/simulator/mod.rs:282-291)."""
+    from cast_simulator_amd import abi
+
+    sv = cs.HipStatevector(n, cs.Precision.F64)
+    sv.zero()
+    nbytes = (1 << n) * 16
+    rows = []
+    for measured in ([0], [n - 1, n // 2, 3], list(range(10)), list(range(n - 10, n)),
+                     list(range(0, n, 2))[:16]):
+        sv.marginal_probabilities(measured)  # warm
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            p = sv.marginal_probabilities(measured)
+            ts.append(time.perf_counter() - t0)
+        t = min(ts)
+        rows.append({"measured": len(measured), "qubits": measured[:4] + (["..."] if len(measured) > 4 else []),
+                     "ms": round(t * 1e3, 3), "gbs": round(nbytes / t / 1e9, 1),
+                     "sum": float(p.sum())})
+    del sv
+    abi.stream_sync(None)
+    return {"workload": f"{n}-qubit f64 statevector (16 GiB), marginals over m measured qubits, "
+                        "synchronous call including the 2^m result copy",
+            "bytes_per_call": nbytes, "calls": rows,
+            "best_hbm_frac": round(max(r["gbs"] for r in rows) / HBM_PEAK_GBS, 4)}
+
+
+def run_density_matrix(cs, n: int = 15, depth: int = 4, p: float = 1e-3, steps: int = 2):
+    """Density-matrix mode: the random-layer circuit on n physical qubits with
+    a depolarizing channel after every single-qubit gate, each gate lifted to
+    its superoperator (2k qubits) and applied to the 2n-qubit vectorised rho
+    (n = 15: 30 virtual qubits, 16 GiB f64).  Per-gate GB/s from the kernel
+    event times, and the trace of rho after the run."""
+    from cast_simulator_amd import abi
+
+    circ = cs.random_layer_circuit(n, depth, SEED)
+    gates = [g.with_depolarizing(p) if g.n_qubits() == 1 else g for g in circ.gates()]
+    lifted = [g.to_density_matrix_gate(n) for g in gates]
+    mgr = cs.HipKernelManager(cs.HipKernelGenSpec.f64())
+    ids = [mgr.generate(g) for g in lifted]
+    rho = cs.HipStatevector(2 * n, cs.Precision.F64)
+    rho.zero()
+    mgr.tune(ids, rho, 4)
+    rho.zero()
+    gpu = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for i in ids:
+            mgr.apply(i, rho)
+        kernels, _ = mgr.sync(True)
+        gpu += sum(t for _, t in kernels)
+    wall = time.perf_counter() - t0
+    nbytes = 2 * (1 << (2 * n)) * 16
+    trace = float(abi.dm_populations(rho.dptr(), n, abi.F64).sum())
+    hist = defaultdict(int)
+    for g in lifted:
+        hist[g.n_qubits()] += 1
+    return {"workload": f"{n}-qubit random-layer circuit, depth {depth}, depolarizing p={p} "
+                        f"after every single-qubit gate; density matrix as a {2 * n}-qubit f64 "
+                        "vector (16 GiB); superoperator gates",
+            "gates": len(ids), "superop_k_histogram": dict(sorted(hist.items())),
+            "steps": steps, "gbs_per_gate": round(len(ids) * steps * nbytes / gpu / 1e9, 1),
+            "wall_gbs_per_gate": round(len(ids) * steps * nbytes / wall / 1e9, 1),
+            "trace_after": trace}
+
+
 def run_trajectories(cs, n: int, count: int, tune_sample: int, plain_sample: int):
     """Batched noisy trajectories (one persistent launch) against the plain
     per-circuit f64 GPU path on the same trajectories.
@@ -456,6 +526,8 @@ def main() -> int:
                          "--workload trajectories")
     ap.add_argument("--no-traj", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the marginal-probability and density-matrix measurements")
     ap.add_argument("--requests", type=int, default=1000)
     ap.add_argument("--rate", type=float, default=200.0)
     ap.add_argument("--traj-qubits", type=int, default=24)
@@ -510,6 +582,12 @@ def main() -> int:
     if not args.no_serve:
         log(f"[bench] serving: {args.requests} requests at {args.rate}/s")
         serve = run_serving(cs, args.requests, args.rate, min(200, args.requests))
+    marg = dm = None
+    if not args.no_extra and args.qubits == 30:
+        log("[bench] marginals on 30 qubits")
+        marg = run_marginals(cs)
+        log("[bench] density matrix on 15 qubits (30 virtual)")
+        dm = run_density_matrix(cs)
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline ...")
@@ -575,6 +653,10 @@ def main() -> int:
         line["trajectories"] = traj
     if serve:
         line["serving"] = serve
+    if marg:
+        line["marginals"] = marg
+    if dm:
+        line["density_matrix"] = dm
     print(json.dumps(line), flush=True)
     return 0
 

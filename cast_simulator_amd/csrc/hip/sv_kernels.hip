@@ -234,23 +234,52 @@ template <typename T> __device__ __forceinline__ double amp_prob(const T *sv, ui
   return __builtin_fma(re, re, im * im);
 }
 
-// Rest-major: workgroup w = c * B + b sums chunk c (chunk_len rest indices,
-// consecutive across the workgroup's threads) of bin b into partial[w].
+// Address-major (n >= 6): every wave reads 64 consecutive amplitudes, lane =
+// address bits 0..5, whether those bits are measured or not.  Workgroup
+// w = c * B_hi + h fixes the measured bits >= 6 to pattern h (deposited in
+// ascending position order) and sums chunk c (chunk_len indices r over the
+// unmeasured bits >= 6, waves interleaved).  Each lane's f64 sum is then
+// reduced over the unmeasured lane bits with xor shuffles, summed over the
+// waves in LDS, and lanes whose unmeasured bits are zero write the bin of
+// their measured low bits and h into partial[c * B + bin].
 template <typename T>
 __global__ void __launch_bounds__(kRedBlock) k_marginal_rest(const T *__restrict__ sv,
                                                              MarginalArgs a, uint64_t chunk_len,
                                                              double *__restrict__ partial) {
-  __shared__ double lds[kRedBlock / 64];
+  __shared__ double lds[kRedBlock];
+  const uint64_t hi_mask = a.mask & ~uint64_t(63);
+  const uint32_t lo_mask = uint32_t(a.mask & 63u);
+  const uint32_t n_hi = uint32_t(__builtin_popcountll(hi_mask));
   const uint64_t w = uint64_t(blockIdx.y) * gridDim.x + blockIdx.x;
-  const uint64_t b = w & ((uint64_t(1) << a.m) - 1), c = w >> a.m;
-  const uint64_t base = scatter_bin(b, a);
+  const uint64_t h = w & ((uint64_t(1) << n_hi) - 1), c = w >> n_hi;
+  uint64_t base = 0; // h deposited at the measured positions >= 6
+  {
+    uint64_t mm = hi_mask;
+    for (uint32_t i = 0; mm; ++i, mm &= mm - 1)
+      base |= ((h >> i) & 1u) << __builtin_ctzll(mm);
+  }
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, n_waves = blockDim.x >> 6;
   double acc = 0.0;
   const uint64_t r0 = c * chunk_len;
-  for (uint64_t r = r0 + threadIdx.x; r < r0 + chunk_len; r += blockDim.x)
-    acc += amp_prob(sv, deposit_rest(r, a.mask) | base);
-  const double t = block_sum(acc, lds);
-  if (threadIdx.x == 0)
-    partial[w] = t;
+  for (uint64_t r = r0 + wave; r < r0 + chunk_len; r += n_waves)
+    acc += amp_prob(sv, deposit_rest(r << 6, hi_mask) | base | lane);
+  for (uint32_t t = 0; t < 6; ++t)
+    if (!((lo_mask >> t) & 1u))
+      acc += __shfl_xor(acc, 1 << t, 64);
+  lds[threadIdx.x] = acc;
+  __syncthreads();
+  if (wave == 0 && (lane & ~lo_mask) == 0) {
+    double s = 0.0;
+    for (uint32_t v = 0; v < n_waves; ++v)
+      s += lds[v * 64 + lane];
+    const uint64_t j = base | lane;
+    uint64_t bin = 0;
+#pragma unroll
+    for (unsigned i = 0; i < kMaxMarginalQubits; ++i)
+      if (i < a.m)
+        bin |= ((j >> a.pos[i]) & 1u) << i;
+    partial[(c << a.m) + bin] = s;
+  }
 }
 
 // Bin-major (few amplitudes per bin): one thread per bin, rest indices in
@@ -501,14 +530,17 @@ void marginal_probabilities(const void *sv, uint32_t n_qubits, uint8_t precision
   const uint64_t n_bins = uint64_t(1) << m, n_rest = uint64_t(1) << (n_qubits - m);
   CallBuffer out(sizeof(double) * n_bins);
   double *d_out = static_cast<double *>(out.p);
-  if (n_rest >= 1024) {
+  const uint32_t n_hi = uint32_t(__builtin_popcountll(a.mask & ~uint64_t(63)));
+  // Indices over the unmeasured positions >= 6 (64 amplitudes each).
+  const uint64_t n_rest_hi = n_qubits >= 6 ? uint64_t(1) << (n_qubits - 6 - n_hi) : 0;
+  if (n_rest_hi >= 16) {
     const uint64_t chunk = std::min<uint64_t>(
-        n_rest, std::max<uint64_t>(1024, (uint64_t(1) << n_qubits) >> 14));
-    const uint64_t n_chunks = n_rest / chunk, blocks = n_bins * n_chunks;
+        n_rest_hi, std::max<uint64_t>(16, (uint64_t(1) << n_qubits) >> 20));
+    const uint64_t n_chunks = n_rest_hi / chunk, blocks = (uint64_t(1) << n_hi) * n_chunks;
     std::unique_ptr<CallBuffer> part;
     double *d_part = d_out;
     if (n_chunks > 1) {
-      part = std::make_unique<CallBuffer>(sizeof(double) * blocks);
+      part = std::make_unique<CallBuffer>(sizeof(double) * n_bins * n_chunks);
       d_part = static_cast<double *>(part->p);
     }
     if (precision == 0)
