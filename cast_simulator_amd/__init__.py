@@ -45,6 +45,9 @@ from ._host import (  # noqa: E402,F401
     NoiseSpec,
     Precision,
     ProfileConfig,
+    QasmAngle,
+    QasmCircuit,
+    QasmGate,
     QuantumCircuit,
     QuantumGate,
     QuantumState,
@@ -72,6 +75,8 @@ from ._host import (  # noqa: E402,F401
     measure_hip,
     optimize,
     optimize_with_log,
+    parse_qasm,
+    qasm_to_quantum_gate,
     poisson_requests,
     random_layer_circuit,
     random_statevector,

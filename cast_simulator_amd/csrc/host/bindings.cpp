@@ -14,6 +14,7 @@
 #include "cost_model.hpp"
 #include "fusion.hpp"
 #include "hip_backend.hpp"
+#include "qasm.hpp"
 #include "serving.hpp"
 #include "trajectory.hpp"
 
@@ -177,13 +178,36 @@ PYBIND11_MODULE(_host, m) {
            })
       .def("n_qubits", &QuantumCircuit::n_qubits)
       .def("__len__", &QuantumCircuit::len)
-      .def_static("qft", &QuantumCircuit::qft, py::arg("n"), py::arg("swap") = true);
+      .def_static("qft", &QuantumCircuit::qft, py::arg("n"), py::arg("swap") = true)
+      .def_static("from_qasm",
+                  [](const std::string &text) { return circuit_from_qasm(parse_qasm(text)); })
+      .def_static("from_openqasm", &circuit_from_qasm);
+
+  py::class_<QasmAngle>(m, "QasmAngle")
+      .def_readonly("pi_multiple", &QasmAngle::pi_multiple)
+      .def_property_readonly("numerator", [](const QasmAngle &a) { return a.frac.num; })
+      .def_property_readonly("denominator", [](const QasmAngle &a) { return a.frac.den; })
+      .def_readonly("number", &QasmAngle::number)
+      .def("radians", &QasmAngle::radians)
+      .def("to_qasm", &QasmAngle::to_qasm);
+  py::class_<QasmGate>(m, "QasmGate")
+      .def("name", &QasmGate::name)
+      .def_readonly("params", &QasmGate::params)
+      .def_readonly("qubits", &QasmGate::qubits)
+      .def("to_qasm", &QasmGate::to_qasm);
+  py::class_<QasmCircuit>(m, "QasmCircuit")
+      .def_readonly("gates", &QasmCircuit::gates)
+      .def("required_qreg_size", &QasmCircuit::required_qreg_size)
+      .def("serialize", &QasmCircuit::serialize);
+  m.def("parse_qasm", &parse_qasm);
+  m.def("qasm_to_quantum_gate", &to_quantum_gate);
   m.def("random_layer_circuit", &random_layer_circuit, py::arg("n"), py::arg("depth"),
         py::arg("seed"));
 
   py::class_<CircuitGraph>(m, "CircuitGraph")
       .def(py::init<>())
       .def_static("from_circuit", &CircuitGraph::from_circuit)
+      .def_static("from_qasm_circuit", &graph_from_qasm)
       .def("n_qubits", &CircuitGraph::n_qubits)
       .def("n_rows", &CircuitGraph::n_rows)
       .def("insert_gate",
