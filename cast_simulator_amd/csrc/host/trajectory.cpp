@@ -577,24 +577,23 @@ TrajectoryTuning TrajectoryEngine::autotune(const std::vector<TrajectorySample> 
     }
   };
   trial(best);
-  // Launch shape: the resident workgroups per CU are bounded by the
-  // occupancy the kernel is built for, so only shapes that can be resident
-  // are tried (slots_per_cu x block <= 512 x waves_per_simd / 2).
+  // Launch shape at the start tuning's occupancy: the resident workgroups
+  // per CU are bounded by it, so only shapes that can be resident are tried
+  // (slots_per_cu x block <= 256 x waves_per_simd).  The 4-waves build is
+  // not searched: it spills registers and was slower in every measured
+  // shape; a caller can still ask for it.
   const TrajectoryTuning start_shape = best;
-  for (uint32_t occ : {2u, 4u})
-    for (uint32_t block : {128u, 256u, 512u})
-      for (uint32_t spc : {1u, 2u, 4u}) {
-        if (spc * block > 256u * occ)
-          continue;
-        TrajectoryTuning t = start_shape;
-        t.waves_per_simd = occ;
-        t.block = block;
-        t.slots_per_cu = spc;
-        if (t.waves_per_simd == start_shape.waves_per_simd && t.block == start_shape.block &&
-            t.slots_per_cu == start_shape.slots_per_cu)
-          continue;
-        trial(t);
-      }
+  for (uint32_t block : {128u, 256u, 512u})
+    for (uint32_t spc : {1u, 2u, 4u}) {
+      if (spc * block > 256u * start_shape.waves_per_simd)
+        continue;
+      TrajectoryTuning t = start_shape;
+      t.block = block;
+      t.slots_per_cu = spc;
+      if (t.block == start_shape.block && t.slots_per_cu == start_shape.slots_per_cu)
+        continue;
+      trial(t);
+    }
   const TrajectoryTuning shape = best;
   for (uint32_t kmax : {3u, 4u})
     for (uint32_t B : {1u, 2u, 4u, 8u}) {

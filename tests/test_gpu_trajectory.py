@@ -231,3 +231,17 @@ def test_abi_rejects_bad_launch_shapes():
     L.precision, L.n_tasks = 1, 4
     with pytest.raises(abi.AbiError, match="null task"):
         abi.traj_run(L)
+
+
+def test_autotuner_returns_a_tried_tuning_that_matches_oracle():
+    n, D = 12, 8
+    lc = cs.LayeredCircuit.random_layers(n, D, 61)
+    S = cs.sample_trajectories(lc, cs.NoiseSpec(0.05, 1, D, 62), 40)
+    pairs = [(i, i + 1) for i in range(n - 1)]
+    eng = cs.TrajectoryEngine(lc, F32, [q for p in pairs for q in p])
+    print("[traj] autotune n=12, 40 trajectories", flush=True)
+    best, trials = eng.autotune(S, cs.TrajectoryTuning(2, 4, 256, 2, 0))
+    assert len(trials) >= 10
+    assert repr(best) in {repr(t) for t, _ in trials}
+    assert all(t.waves_per_simd == 2 and t.slots_per_cu * t.block <= 512 for t, _ in trials)
+    check_run(lc, S, F32, best, pairs)
