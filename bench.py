@@ -33,7 +33,8 @@ METRIC = "HBM GB/s per fused gate at 30q f64; batched circuits/s + p99 at fixed 
 HBM_PEAK_GBS = 8000.0
 SEED = 20260101
 KERNEL = {1: "k_dense", 2: "k_monomial", 3: "k_tile", 4: "k_mfma", 5: "k_tile(swizzled)",
-          6: "k_tile(3 products)", 7: "k_tile(3 products, swizzled)"}
+          6: "k_tile(3 products)", 7: "k_tile(3 products, swizzled)", 8: "k_tile(xcd order)",
+          9: "k_tile(3 products, xcd order)"}
 
 
 def log(msg: str) -> None:
@@ -76,7 +77,7 @@ def run_gpu(cs, n: int, precision: str, steps: int, warmup: int, max_size: int):
     # Setup (untimed, like the reference's compile phase): per-gate kernel
     # autotuning on the resident statevector.
     t_tune = time.perf_counter()
-    rounds = mgr.tune(ids, sv, 12)
+    rounds = mgr.tune(ids, sv, 32)
     t_tune = time.perf_counter() - t_tune
     fam = {i: mgr.family(i, n) for i in set(ids)}
     log(f"  autotune: {rounds} passes, {t_tune:.1f} s")
@@ -91,9 +92,9 @@ def run_gpu(cs, n: int, precision: str, steps: int, warmup: int, max_size: int):
     def symbol(kid):
         """Kernel symbol as the profiler names it."""
         k, f = info[kid].k, fam[kid]
-        if f in (3, 5, 6, 7):
+        if f in (3, 5, 6, 7, 8, 9):
             low = sum(1 for q in info[kid].qubits if q < 6)
-            return (f"k_tile3<{ctype}, {k}, {low}>" if f in (6, 7)
+            return (f"k_tile3<{ctype}, {k}, {low}>" if f in (6, 7, 9)
                     else f"k_tile<{ctype}, {k}, {low}>")
         return f"{KERNEL[f]}<{ctype}, {k}>"
 
@@ -275,7 +276,7 @@ def run_density_matrix(cs, n: int = 15, depth: int = 4, p: float = 1e-3, steps: 
     ids = [mgr.generate(g) for g in lifted]
     rho = cs.HipStatevector(2 * n, cs.Precision.F64)
     rho.zero()
-    mgr.tune(ids, rho, 4)
+    mgr.tune(ids, rho, 16)
     rho.zero()
     gpu = 0.0
     t0 = time.perf_counter()

@@ -41,6 +41,7 @@ class KernelGenSpec(C.Structure):
 
 FLAG_TILE_SWIZZLE = 1
 FLAG_TILE_GAUSS = 2
+FLAG_TILE_XCD = 4
 
 
 class PlanInfo(C.Structure):

@@ -166,7 +166,7 @@ GatePlan *build_plan(const cast_hip_kernel_gen_spec_t *spec, const cast_complex6
     variant = CAST_HIP_VARIANT_DENSE; // matrix-core tiles exist for k = 4, 5, 6
   plan->variant = variant;
   plan->forced = spec->variant != CAST_HIP_VARIANT_AUTO;
-  if (spec->flags & ~(CAST_HIP_FLAG_TILE_SWIZZLE | CAST_HIP_FLAG_TILE_GAUSS))
+  if (spec->flags & ~(CAST_HIP_FLAG_TILE_SWIZZLE | CAST_HIP_FLAG_TILE_GAUSS | CAST_HIP_FLAG_TILE_XCD))
     throw Error("unknown flags " + std::to_string(spec->flags));
   plan->flags = spec->flags;
 

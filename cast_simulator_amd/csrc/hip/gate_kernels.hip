@@ -76,7 +76,24 @@ struct TileGeom {
   uint32_t reg_pos[6]; // address bit of register bit b during load/store
   uint32_t low_t[6];   // targets below 6, ascending
   uint32_t swz;        // 0, or the number of low tile-index bits visited bit-reversed
+  uint32_t xcd_bits;   // 0, or log2(blocks): block b runs the tiles of block
+                       // (b mod 8) * (blocks / 8) + b / 8 (contiguous per XCD)
 };
+
+// Tile visited by the wave `tile` (its launch-order index) under the
+// geometry's orders; both are bijections on [0, n_tiles).
+__device__ __forceinline__ uint64_t visit_order(uint64_t tile, const TileGeom &g) {
+  if (g.xcd_bits) { // 4 waves per block, blocks = 2^xcd_bits >= 8
+    const uint64_t b = tile >> 2;
+    const uint64_t rb = ((b & 7u) << (g.xcd_bits - 3)) | (b >> 3);
+    tile = (rb << 2) | (tile & 3u);
+  }
+  if (g.swz) { // n_tiles is a power of two >= 2^swz
+    const uint64_t lo = tile & ((uint64_t(1) << g.swz) - 1);
+    tile = (tile ^ lo) | (__brevll(lo) >> (64 - g.swz));
+  }
+  return tile;
+}
 
 template <typename T, int KL, int L>
 __global__ void __launch_bounds__(256) k_tile(T *__restrict__ sv, const T *__restrict__ mat,
@@ -86,10 +103,7 @@ __global__ void __launch_bounds__(256) k_tile(T *__restrict__ sv, const T *__res
       ((uint64_t(blockIdx.y) * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
   if (tile >= g.n_tiles)
     return; // whole wave
-  if (g.swz) { // bijection on [0, n_tiles): n_tiles is a power of two >= 2^swz
-    const uint64_t lo = tile & ((uint64_t(1) << g.swz) - 1);
-    tile = (tile ^ lo) | (__brevll(lo) >> (64 - g.swz));
-  }
+  tile = visit_order(tile, g);
   uint64_t tb = tile;
 #pragma unroll
   for (int b = 0; b < KL; ++b)
@@ -111,10 +125,7 @@ __global__ void __launch_bounds__(256, 2) k_tile3(T *__restrict__ sv, const T *_
       ((uint64_t(blockIdx.y) * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
   if (tile >= g.n_tiles)
     return; // whole wave
-  if (g.swz) {
-    const uint64_t lo = tile & ((uint64_t(1) << g.swz) - 1);
-    tile = (tile ^ lo) | (__brevll(lo) >> (64 - g.swz));
-  }
+  tile = visit_order(tile, g);
   uint64_t tb = tile;
 #pragma unroll
   for (int b = 0; b < KL; ++b)
@@ -484,6 +495,9 @@ TileGeom tile_geom(const GatePlan &p, uint32_t n, uint32_t *n_low) {
   const uint32_t L = t.n_low;
   if (p.flags & CAST_HIP_FLAG_TILE_SWIZZLE)
     tg.swz = std::min<uint32_t>(10, n - 6 - p.k);
+  // blocks = n_tiles / 4 (4 waves per block, one tile each), a power of two
+  if ((p.flags & CAST_HIP_FLAG_TILE_XCD) && tg.n_tiles >= 32)
+    tg.xcd_bits = n - 6 - p.k - 2;
   *n_low = L;
   return tg;
 }

@@ -165,6 +165,8 @@ HipKernelManager::TuneState &HipKernelManager::tune_state_locked(KernelId id, ui
       {CAST_HIP_VARIANT_TILE, CAST_HIP_FLAG_TILE_SWIZZLE},
       {CAST_HIP_VARIANT_TILE, CAST_HIP_FLAG_TILE_GAUSS},
       {CAST_HIP_VARIANT_TILE, CAST_HIP_FLAG_TILE_GAUSS | CAST_HIP_FLAG_TILE_SWIZZLE},
+      {CAST_HIP_VARIANT_TILE, CAST_HIP_FLAG_TILE_XCD},
+      {CAST_HIP_VARIANT_TILE, CAST_HIP_FLAG_TILE_GAUSS | CAST_HIP_FLAG_TILE_XCD},
       {CAST_HIP_VARIANT_MFMA, 0u},
       {CAST_HIP_VARIANT_DENSE, 0u}};
   for (const auto &c : cands) {
@@ -183,9 +185,11 @@ HipKernelManager::TuneState &HipKernelManager::tune_state_locked(KernelId id, ui
     // Candidate code: the family, or one of the wave-tile variants.
     const bool gauss = c.second & CAST_HIP_FLAG_TILE_GAUSS;
     const bool swz = c.second & CAST_HIP_FLAG_TILE_SWIZZLE;
+    const bool xcd = c.second & CAST_HIP_FLAG_TILE_XCD;
     uint32_t code = fam;
     if (fam == CAST_HIP_VARIANT_TILE && c.second)
-      code = gauss ? (swz ? kTileGaussSwizzled : kTileGauss) : kTileSwizzled;
+      code = xcd ? (gauss ? kTileGaussXcd : kTileXcd)
+                 : gauss ? (swz ? kTileGaussSwizzled : kTileGauss) : kTileSwizzled;
     const bool gauss_ok = g.n_qubits() == 4 || g.n_qubits() == 5;
     if (!ok || (c.second && fam != CAST_HIP_VARIANT_TILE) || (gauss && !gauss_ok) ||
         std::find(st.fams.begin(), st.fams.end(), code) != st.fams.end()) {
@@ -196,6 +200,7 @@ HipKernelManager::TuneState &HipKernelManager::tune_state_locked(KernelId id, ui
     st.plans.push_back(plan);
   }
   st.samples.resize(st.fams.size());
+  st.dropped.assign(st.fams.size(), 0);
   if (st.fams.size() <= 1)
     st.chosen = 0;
   return tune_.emplace(key, std::move(st)).first->second;
@@ -261,9 +266,11 @@ void HipKernelManager::apply(KernelId id, HipStatevector &sv) {
     if (st.chosen >= 0) {
       plan = st.plans[size_t(st.chosen)];
     } else {
-      size_t best = 0;
-      for (size_t c = 1; c < st.samples.size(); ++c)
-        if (st.samples[c].size() < st.samples[best].size())
+      // the surviving candidate with the fewest timings
+      size_t best = st.samples.size();
+      for (size_t c = 0; c < st.samples.size(); ++c)
+        if (!st.dropped[c] &&
+            (best == st.samples.size() || st.samples[c].size() < st.samples[best].size()))
           best = c;
       cand = int(best);
       plan = st.plans[best];
@@ -387,19 +394,30 @@ SyncStats HipKernelManager::sync(bool timed) {
         continue;
       TuneState &ts = it->second;
       ts.samples[size_t(work[i].cand)].push_back(st.kernels[i].gpu_s);
-      bool done = true;
-      for (const auto &s : ts.samples)
-        done = done && s.size() >= tune_samples_;
-      if (done) {
-        double best = 1e300;
-        for (size_t c = 0; c < ts.samples.size(); ++c) {
+      // best minimum so far; drop candidates beyond the margin
+      double best = 1e300;
+      size_t arg = 0;
+      for (size_t c = 0; c < ts.samples.size(); ++c)
+        if (!ts.dropped[c] && !ts.samples[c].empty()) {
           const double m = *std::min_element(ts.samples[c].begin(), ts.samples[c].end());
           if (m < best) {
             best = m;
-            ts.chosen = int(c);
+            arg = c;
           }
         }
+      bool done = true;
+      for (size_t c = 0; c < ts.samples.size(); ++c) {
+        if (ts.dropped[c])
+          continue;
+        if (!ts.samples[c].empty() && c != arg &&
+            *std::min_element(ts.samples[c].begin(), ts.samples[c].end()) > kTuneMargin * best) {
+          ts.dropped[c] = 1;
+          continue;
+        }
+        done = done && ts.samples[c].size() >= tune_samples_;
       }
+      if (done)
+        ts.chosen = int(arg);
     }
   }
   st.wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - w0).count();

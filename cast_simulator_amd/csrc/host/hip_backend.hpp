@@ -95,9 +95,13 @@ struct PlanInfo {
 };
 
 // Per-gate kernel autotuning: for a dense gate on a statevector of at least
-// `min_qubits` qubits, every applicable kernel family (wave tile, matrix
-// core, direct) is timed on the real gate, `samples` times each, and the
+// `min_qubits` qubits, every applicable kernel (wave tile in three orders and
+// two product forms, matrix core, direct) is timed on the real gate, and the
 // fastest (by minimum) is used from then on.  Keyed by (kernel id, n).
+// Successive halving: every candidate is timed once; candidates slower than
+// kTuneMargin x the best so far are dropped; the rest are timed until each
+// has `samples` timings.
+constexpr double kTuneMargin = 1.08;
 // Candidate code of the wave-tile kernel visiting tiles in swizzled order
 // (the other codes are the CAST_HIP_VARIANT_* families).
 constexpr uint32_t kTileSwizzled = 5;
@@ -105,6 +109,9 @@ constexpr uint32_t kTileSwizzled = 5;
 // in natural and swizzled order.
 constexpr uint32_t kTileGauss = 6;
 constexpr uint32_t kTileGaussSwizzled = 7;
+// ... and both wave-tile forms in XCD-blocked order (CAST_HIP_FLAG_TILE_XCD).
+constexpr uint32_t kTileXcd = 8;
+constexpr uint32_t kTileGaussXcd = 9;
 
 struct TuneEntry {
   KernelId id = 0;
@@ -154,6 +161,7 @@ private:
     std::vector<uint32_t> fams;
     std::vector<void *> plans;
     std::vector<std::vector<double>> samples;
+    std::vector<char> dropped;
     int chosen = -1;
   };
   void ensure_stream_locked();

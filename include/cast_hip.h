@@ -77,6 +77,11 @@ typedef struct cast_hip_kernel_gen_spec_t {
  * per output: Re = Mr vr - Mi vi, Im = (Mr + Mi)(vr + vi) - Mr vr - Mi vi),
  * for gates where the FMA rate, not HBM, limits the wave tiles. */
 #define CAST_HIP_FLAG_TILE_GAUSS 2u
+/* Visit wave tiles so that the blocks dispatched to each of the 8 XCDs
+ * (block b runs on XCD b mod 8) cover one contiguous eighth of the tiles,
+ * instead of every XCD interleaving with the others.  Faster for some
+ * target sets and slower for others; the host autotuner tries it. */
+#define CAST_HIP_FLAG_TILE_XCD 4u
 
 /* AUTO: monomial when every row has <= 1 nonzero, else the dense family, whose
  * kernel (direct / wave tile / matrix core) is picked per launch from k, the

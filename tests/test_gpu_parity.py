@@ -190,20 +190,24 @@ def apply_abi(matrix, qubits, init, f32, variant, n, flags=0):
 VARIANTS = [(abi.VARIANT_AUTO, 0), (abi.VARIANT_DENSE, 0), (abi.VARIANT_TILE, 0),
             (abi.VARIANT_TILE, abi.FLAG_TILE_SWIZZLE), (abi.VARIANT_TILE, abi.FLAG_TILE_GAUSS),
             (abi.VARIANT_TILE, abi.FLAG_TILE_GAUSS | abi.FLAG_TILE_SWIZZLE),
+            (abi.VARIANT_TILE, abi.FLAG_TILE_XCD),
+            (abi.VARIANT_TILE, abi.FLAG_TILE_GAUSS | abi.FLAG_TILE_XCD),
             (abi.VARIANT_MFMA, 0)]
 
 
 @pytest.mark.parametrize("variant,flags", VARIANTS,
                          ids=["auto", "direct", "tile", "tile-swizzled", "tile-gauss",
-                              "tile-gauss-swizzled", "mfma"])
+                              "tile-gauss-swizzled", "tile-xcd", "tile-gauss-xcd", "mfma"])
 @pytest.mark.parametrize("f32", [False, True], ids=["f64", "f32"])
 def test_every_kernel_family_every_shape(f32, variant, flags):
     """Each dense kernel family, forced through the C ABI, on every target
-    shape (low/high/mixed targets, k = 1..6) of a 14-qubit statevector and of
-    the smallest statevector each family accepts."""
+    shape (low/high/mixed targets, k = 1..6) of a 14-qubit statevector (16
+    for the XCD-blocked order, which needs at least 32 tiles) and of the
+    smallest statevector each family accepts."""
+    big = 16 if flags & abi.FLAG_TILE_XCD else 14
     for qubits in SHAPES:
         g = cs.QuantumGate.random_unitary(qubits, 300 + sum(qubits))
-        for n in (14, len(qubits) + 6, max(qubits) + 1):
+        for n in (big, len(qubits) + 6, max(qubits) + 1):
             if max(qubits) >= n:
                 continue
             init = seeded_state(n)
@@ -437,8 +441,10 @@ def test_autotuned_kernels_match_oracle(f32):
     scratch.zero()
     assert mgr.tune(ids, scratch) >= 1
     report = mgr.tuning_report()
-    # chosen: direct (1), wave tile (3), matrix core (4), swizzled wave tile (5)
-    assert len(report) == len(ids) and all(e["chosen"] in (1, 3, 4, 5, 6, 7) for e in report)
+    # chosen: direct (1), wave tile (3), matrix core (4), wave tile swizzled (5),
+    # three products (6), both (7), XCD order (8), three products in XCD order (9)
+    assert len(report) == len(ids) and all(e["chosen"] in (1, 3, 4, 5, 6, 7, 8, 9)
+                                           for e in report)
     init = np.asarray(cs.random_statevector(n, 3))
     sv = cs.HipStatevector(n, prec(f32))
     sv.upload(init)
