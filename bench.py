@@ -110,6 +110,19 @@ def run_gpu(cs, n: int, precision: str, steps: int, warmup: int, max_size: int):
     wall = time.perf_counter() - t0
     scalar = 8 if precision == "f64" else 4
     bytes_per_gate = 2 * (1 << n) * 2 * scalar
+    # Streaming ceiling on the same statevector: the in-place scale kernel
+    # (16-byte loads and stores, no gather) reads and writes every amplitude
+    # once, the same bytes as a gate; best of 5 synchronous calls.
+    from cast_simulator_amd import abi
+
+    pcode = abi.F64 if precision == "f64" else abi.F32
+    abi.scale(sv.dptr(), n, pcode, 1.0)
+    ceil_t = []
+    for _ in range(5):
+        c0 = time.perf_counter()
+        abi.scale(sv.dptr(), n, pcode, 1.0)
+        ceil_t.append(time.perf_counter() - c0)
+    ceiling_gbs = bytes_per_gate / min(ceil_t) / 1e9
     n_gates = len(ids)
     gbs = n_gates * steps * bytes_per_gate / wall / 1e9
     # dominant kernel = the kernel symbol with the most GPU time
@@ -137,6 +150,7 @@ def run_gpu(cs, n: int, precision: str, steps: int, warmup: int, max_size: int):
         "per_k_gbs": {s: bytes_per_gate / (sum(t) / len(t)) / 1e9
                       for s, t in sorted(per_kernel.items())},
         "norm_after": norm,
+        "stream_ceiling_gbs": ceiling_gbs,
         "profile": src,
         "crossover_ai": prof.crossover_ai,
     }
@@ -636,6 +650,8 @@ def main() -> int:
             "traffic_source": "profiles/gate_f64_30q_traffic.json" if traffic else None,
             "avg_launch_ms": round(dom["avg_ms"], 4),
             "launches": dom["launches"],
+            "stream_ceiling_gbs": round(r64["stream_ceiling_gbs"], 1),
+            "value_frac_of_stream_ceiling": round(r64["gbs_per_gate"] / r64["stream_ceiling_gbs"], 4),
         },
         "per_kernel_gbs": {k: round(v, 1) for k, v in r64["per_k_gbs"].items()},
         "autotune": r64["autotune"],

@@ -61,12 +61,20 @@ template <typename F> int guard(char *err, size_t len, F &&f) {
   return 1;
 }
 
+// Every statevector kernel moves 16-byte vectors (one f64 amplitude, two f32
+// ones); allocations from cast_hip_sv_alloc are 256-byte aligned.
+void check_aligned(uint64_t dptr) {
+  if (dptr % 16 != 0)
+    throw Error("statevector pointer must be 16-byte aligned");
+}
+
 void check_sv_args(uint64_t dptr, size_t n_elements, uint8_t precision) {
   check_precision(precision);
   if (dptr == 0)
     throw Error("null device pointer");
   if (n_elements == 0)
     throw Error("n_elements must be positive");
+  check_aligned(dptr);
 }
 
 // Matrix scalar classification of src/cpp/cuda/cuda_gen.cpp:67-112: the
@@ -619,6 +627,7 @@ int cast_hip_gate_apply(const void *plan, void *stream, uint64_t sv_dptr, uint32
       throw Error("null plan");
     if (!sv_dptr)
       throw Error("null statevector pointer");
+    check_aligned(sv_dptr);
     launch_gate(*static_cast<const GatePlan *>(plan), static_cast<hipStream_t>(stream),
                 reinterpret_cast<void *>(sv_dptr), sv_n_qubits);
   });
@@ -644,6 +653,7 @@ int cast_hip_gate_apply_batch(const void *const *plans, const uint64_t *svs,
         throw Error("batched plans must share k and precision");
       if (!svs[i])
         throw Error("null statevector at index " + std::to_string(i));
+      check_aligned(svs[i]);
       const uint32_t n = sv_n_qubits[i];
       if (n < p->k || p->q[p->k - 1] >= n || n > 40)
         throw Error("item " + std::to_string(i) + ": gate does not fit a " +
@@ -668,6 +678,7 @@ int cast_hip_expect_zz(uint64_t sv_dptr, uint32_t sv_n_qubits, uint8_t precision
     check_precision(precision);
     if (!sv_dptr)
       throw Error("null statevector pointer");
+    check_aligned(sv_dptr);
     if (n_pairs > 0 && (!pairs || !out))
       throw Error("null pairs/out");
     if (sv_n_qubits > 40)
@@ -697,6 +708,7 @@ int cast_hip_observe_async(uint64_t sv_dptr, uint32_t sv_n_qubits, uint8_t preci
     check_precision(precision);
     if (!sv_dptr || !out || !scratch)
       throw Error("null statevector, output or scratch pointer");
+    check_aligned(sv_dptr);
     if (n_pairs > 32)
       throw Error("at most 32 qubit pairs");
     if (n_pairs > 0 && !pairs)
@@ -720,6 +732,8 @@ int cast_hip_inner_product(uint64_t a, uint8_t precision_a, uint64_t b, uint8_t 
     check_precision(precision_b);
     if (!a || !b || !out_re_im)
       throw Error("null statevector or output pointer");
+    check_aligned(a);
+    check_aligned(b);
     if (sv_n_qubits < 1 || sv_n_qubits > 40)
       throw Error("statevector qubit count out of range");
     inner_product(reinterpret_cast<const void *>(a), precision_a, reinterpret_cast<const void *>(b),
@@ -734,6 +748,7 @@ int cast_hip_marginal_probabilities(uint64_t sv_dptr, uint32_t sv_n_qubits, uint
     check_precision(precision);
     if (!sv_dptr || !out)
       throw Error("null statevector or output pointer");
+    check_aligned(sv_dptr);
     if (sv_n_qubits < 1 || sv_n_qubits > 40)
       throw Error("statevector qubit count out of range");
     if (n_measured && !measured_qubits)
@@ -752,6 +767,7 @@ int cast_hip_dm_populations(uint64_t sv_dptr, uint32_t n_physical, uint8_t preci
     check_precision(precision);
     if (!sv_dptr || !out)
       throw Error("null statevector or output pointer");
+    check_aligned(sv_dptr);
     if (n_physical < 1 || n_physical > 20)
       throw Error("density-matrix qubit count out of range (1..20)");
     dm_populations(reinterpret_cast<const void *>(sv_dptr), n_physical, precision, out,

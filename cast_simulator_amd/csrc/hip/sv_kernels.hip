@@ -94,11 +94,26 @@ __global__ void __launch_bounds__(kRedBlock) k_sum_partials(const double *__rest
     out[blockIdx.x] = s;
 }
 
+// In-place scale: one 16-byte vector per thread over a grid that covers the
+// whole statevector (block b, thread t: vector b * blockDim + t), the access
+// pattern of a one-tile-per-wave gate launch; a scalar tail, if any, is done
+// by the first threads.
 template <typename T>
 __global__ void __launch_bounds__(kRedBlock) k_scale(T *__restrict__ x, size_t n, double scale) {
-  const size_t stride = size_t(gridDim.x) * blockDim.x;
-  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
-    x[i] = T(double(x[i]) * scale);
+  constexpr int VW = 16 / sizeof(T);
+  using VT = T __attribute__((ext_vector_type(16 / sizeof(T))));
+  const size_t i = (size_t(blockIdx.y) * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x;
+  const size_t nv = n / VW;
+  if (i < nv) {
+    VT *xv = reinterpret_cast<VT *>(x);
+    VT v = __builtin_nontemporal_load(xv + i);
+#pragma unroll
+    for (int j = 0; j < VW; ++j)
+      v[j] = T(double(v[j]) * scale);
+    __builtin_nontemporal_store(v, xv + i);
+  }
+  if (i < n - nv * VW)
+    x[nv * VW + i] = T(double(x[nv * VW + i]) * scale);
 }
 
 struct PairArgs {
@@ -373,12 +388,17 @@ double norm_squared(const void *sv, size_t n_elements, uint8_t precision, hipStr
 
 void launch_scale(void *sv, size_t n_elements, uint8_t precision, double scale,
                   hipStream_t stream) {
-  const unsigned grid = red_grid(n_elements) * 4;
+  const size_t vw = precision == 0 ? 4 : 2;
+  const uint64_t threads = std::max<uint64_t>(n_elements / vw, n_elements % vw);
+  const uint64_t blocks = (threads + kRedBlock - 1) / kRedBlock;
+  const uint64_t cap = uint64_t(1) << 20;
+  const dim3 grid = blocks <= cap ? dim3(unsigned(blocks))
+                                  : dim3(unsigned(cap), unsigned((blocks + cap - 1) / cap));
   if (precision == 0)
-    hipLaunchKernelGGL(k_scale<float>, dim3(grid), dim3(kRedBlock), 0, stream,
+    hipLaunchKernelGGL(k_scale<float>, grid, dim3(kRedBlock), 0, stream,
                        static_cast<float *>(sv), n_elements, scale);
   else
-    hipLaunchKernelGGL(k_scale<double>, dim3(grid), dim3(kRedBlock), 0, stream,
+    hipLaunchKernelGGL(k_scale<double>, grid, dim3(kRedBlock), 0, stream,
                        static_cast<double *>(sv), n_elements, scale);
   check(hipGetLastError(), "hipLaunchKernelGGL(k_scale)");
 }

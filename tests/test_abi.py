@@ -125,3 +125,16 @@ def test_measurement_entry_points_validate_before_the_device():
     assert b"1..20" in buf.value
     assert L.cast_hip_dm_populations(64, 4, 1, None, None, buf, abi.ERR_LEN) == 1
     assert b"null" in buf.value
+
+
+def test_misaligned_statevector_pointers_are_rejected():
+    buf = C.create_string_buffer(abi.ERR_LEN)
+    L = abi.lib()
+    assert L.cast_hip_scale(0x1008, 8, 1, 1.0, buf, abi.ERR_LEN) == 1
+    assert b"16-byte aligned" in buf.value
+    out = (C.c_double * 2)()
+    q = (C.c_uint32 * 1)(0)
+    assert L.cast_hip_marginal_probabilities(0x1004, 8, 1, q, 1, out, None, buf, abi.ERR_LEN) == 1
+    assert b"16-byte aligned" in buf.value
+    assert L.cast_hip_inner_product(0x1000, 1, 0x1008, 1, 4, out, None, buf, abi.ERR_LEN) == 1
+    assert b"16-byte aligned" in buf.value
