@@ -488,12 +488,14 @@ PYBIND11_MODULE(_host, m) {
              return s.sample_trajectory(g, o);
            })
       .def("bench",
-           [](Simulator &s, const CircuitGraph &g, double budget) {
-             const auto r = s.bench(g, budget);
+           [](Simulator &s, const CircuitGraph &g, double budget, Representation repr) {
+             const auto r = s.bench(g, budget, repr);
              return py::dict(py::arg("compile_s") = r.compile_s,
                              py::arg("samples_s") = r.samples_s, py::arg("n_gates") = r.n_gates,
                              py::arg("mean_s") = r.exec.mean_s, py::arg("min_s") = r.exec.min_s);
-           })
+           },
+           py::arg("graph"), py::arg("budget_s"),
+           py::arg("repr") = Representation::StateVector)
       .def("manager", &Simulator::manager, py::return_value_policy::reference_internal);
 
   m.def("measure_hip",

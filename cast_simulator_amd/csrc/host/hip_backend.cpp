@@ -508,13 +508,23 @@ HipStatevector Simulator::simulate(const CircuitGraph &graph) {
   return sv;
 }
 
-Simulator::BenchResult Simulator::bench(const CircuitGraph &graph, double budget_s) {
+Simulator::BenchResult Simulator::bench(const CircuitGraph &graph, double budget_s,
+                                        Representation repr) {
   BenchResult r;
   const auto t0 = std::chrono::steady_clock::now();
-  const auto ids = compile(graph);
+  std::vector<KernelId> ids;
+  uint32_t n_sv = uint32_t(graph.n_qubits());
+  if (repr == Representation::StateVector) {
+    ids = compile(graph);
+  } else { // prepare_gates, src/simulator/mod.rs:607-615
+    const uint32_t n = n_sv;
+    n_sv = 2 * n;
+    for (const auto &g : graph.gates_in_row_order())
+      ids.push_back(mgr_.generate(g->to_density_matrix_gate(n)));
+  }
   r.compile_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   r.n_gates = ids.size();
-  HipStatevector sv(uint32_t(graph.n_qubits()), spec_.precision);
+  HipStatevector sv(n_sv, spec_.precision);
   r.samples_s = time_adaptive_samples_with(
       [&] {
         sv.zero();

@@ -273,7 +273,10 @@ public:
     TimingStats exec;
     size_t n_gates = 0;
   };
-  BenchResult bench(const CircuitGraph &graph, double budget_s);
+  // src/simulator/bench.rs:136-167: compile once, then time repeated
+  // executions (reset + every gate); DensityMatrix lifts the gates first.
+  BenchResult bench(const CircuitGraph &graph, double budget_s,
+                    Representation repr = Representation::StateVector);
 
 private:
   HipKernelGenSpec spec_;

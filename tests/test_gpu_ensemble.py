@@ -211,3 +211,14 @@ def test_ensemble_matches_oracle_ensemble(M):
     assert np.abs(obs - mix).max() < 5 * np.sqrt(0.25 / N)
     again = cs.Simulator.f64().sample_trajectory(graph_of(n, gl), opts(measured, N, M, 7))
     assert again.histogram == t.histogram
+
+
+def test_bench_times_density_matrix_runs():
+    g = graph_of(5, noisy_circuit(5, 3))
+    sim = cs.Simulator.f64()
+    print("[dm] bench n=5", flush=True)
+    r = sim.bench(g, 0.05, cs.Representation.DensityMatrix)
+    assert r["n_gates"] == len(g.gates_in_row_order()) and len(r["samples_s"]) >= 1
+    assert all(t > 0 for t in r["samples_s"])
+    r2 = sim.bench(graph_of(5, [cs.QuantumGate.h(0)]), 0.02)
+    assert r2["n_gates"] == 1
