@@ -222,3 +222,33 @@ def test_bench_times_density_matrix_runs():
     assert all(t > 0 for t in r["samples_s"])
     r2 = sim.bench(graph_of(5, [cs.QuantumGate.h(0)]), 0.02)
     assert r2["n_gates"] == 1
+
+
+@pytest.mark.parametrize("n", [1, 3, 5])
+def test_marginal_small_statevectors(n):
+    sv, x = uploaded(n, F64, 500 + n)
+    print(f"[marginal] small n={n}", flush=True)
+    for measured in ([], [0], list(range(n)), list(reversed(range(n)))):
+        got = sv.marginal_probabilities(measured)
+        assert np.allclose(got, oracle.marginal(x, measured), rtol=1e-13, atol=1e-16)
+
+
+def test_density_matrix_of_one_qubit():
+    g = graph_of(1, [cs.QuantumGate.h(0), cs.QuantumGate.depolarizing(0, 0.3)])
+    st = cs.Simulator.f64().simulate(g, cs.Representation.DensityMatrix)
+    assert np.allclose(st.populations(), [0.5, 0.5], atol=1e-14)
+    amps = st.download_amplitudes()
+    # rho = 0.5 I + (1 - 4p/3) / 2 X for H|0> under depolarizing p
+    rho01 = amps[2 * (0 | (1 << 1))]
+    assert abs(rho01 - 0.5 * (1 - 4 * 0.3 / 3)) < 1e-14
+
+
+def test_trajectory_edge_cases():
+    sim = cs.Simulator.f64()
+    g = graph_of(3, [cs.QuantumGate.h(0), cs.QuantumGate.depolarizing(0, 0.2)])
+    t = sim.sample_trajectory(g, opts([], 500, 4))
+    assert t.histogram == {0: 500}                        # no measured qubits: one outcome
+    t = sim.sample_trajectory(g, opts([0, 2], 0, 4))
+    assert t.histogram == {} and [b.n_samples for b in t.branches] == [0, 0, 0, 0]
+    with pytest.raises(cs.CastError, match="out of range"):
+        sim.sample_trajectory(g, opts([3], 10, 1))
