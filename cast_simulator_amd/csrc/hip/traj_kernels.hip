@@ -7,9 +7,11 @@
 // every later gate updates the slot in place, with a workgroup barrier
 // between gates.  Gates use wave tiles (coalesced 64-amplitude rows, cross-
 // lane exchange of targets below bit 6) when n >= k + 6, else one combo per
-// thread.  A wave keeps 16 amplitude loads in flight per lane whatever k is
-// (16 / K tiles per step), since one workgroup alone must cover the memory
-// latency of its statevector.  At the end the workgroup reduces, in f64, the
+// thread.  A wave keeps at least 16 amplitude loads in flight per lane
+// whatever k is (16 / K tiles per step for k < 4, one tile of K for k = 4, 5),
+// since one workgroup alone must cover the memory latency of its
+// statevector.  Gates of k = 4 and 5 (k = 5 in f32 only) use the
+// three-product form: the pool holds [Re M | Im M | Re M + Im M].  At the end the workgroup reduces, in f64, the
 // squared norm and sum |amp|^2 (-1)^(bit a xor bit b) for each requested
 // pair (a, b), and can copy the final state out.
 //
@@ -33,7 +35,7 @@ using dev::V2;
 constexpr int kMaxPairs = 32;
 constexpr int kMaxBlock = 512;
 constexpr int kMaxWaves = kMaxBlock / 64;
-constexpr uint32_t kMaxTrajGateQubits = 4;
+constexpr uint32_t kMaxTrajGateQubits = 5; // 4 at f64
 
 // Layouts the host bindings rely on (no padding surprises across compilers).
 static_assert(sizeof(cast_hip_traj_gate_t) == 12, "cast_hip_traj_gate_t layout");
@@ -108,13 +110,13 @@ __device__ __forceinline__ void wg_tile_gate(const T *src, T *dst, const T *__re
         s[u] = src + off;
         d[u] = dst + off;
       }
-      dev::tile_step<T, KL, L, U, KL == 4>(s, d, m, reg_pos, low_t, lane);
+      dev::tile_step<T, KL, L, U, (KL >= 4)>(s, d, m, reg_pos, low_t, lane);
     }
   } else {
     for (uint64_t t = wave; t < n_tiles; t += n_waves) {
       const T *m = mat + opaque_u32(0);
       const uint64_t off = tile_ptr(t);
-      dev::tile_update<T, KL, L, KL == 4>(src + off, dst + off, m, reg_pos, low_t, lane);
+      dev::tile_update<T, KL, L, (KL >= 4)>(src + off, dst + off, m, reg_pos, low_t, lane);
     }
   }
 }
@@ -135,7 +137,7 @@ __device__ __forceinline__ void wg_dense_gate(const T *src, T *dst, const T *__r
     const VT *in = reinterpret_cast<const VT *>(src);
     VT *out = reinterpret_cast<VT *>(dst);
     const uint64_t base = combo_base<KL>(c, q);
-    if constexpr (KL == 4) { // three real matrices [Re | Im | Re + Im]
+    if constexpr (KL >= 4) { // three real matrices [Re | Im | Re + Im]
       T vr[K], vi[K];
 #pragma unroll
       for (int i = 0; i < K; ++i) {
@@ -172,6 +174,14 @@ __device__ __forceinline__ void wg_gate_k(const T *src, T *dst, const T *__restr
 #pragma unroll
   for (int b = 0; b < KL; ++b)
     n_low += q[b] < 6;
+  if constexpr (KL == 5) {
+    // Exchanging three or more target bits across lanes for K = 32 needs
+    // more registers than the kernel's budget (it would spill in every
+    // path); such gates, rare at this size, take the one-combo-per-thread
+    // path.
+    if (n_low > 2)
+      return wg_dense_gate<T, KL>(src, dst, mat, q, n);
+  }
   switch (n_low) {
   case 0:
     return wg_tile_gate<T, KL, 0>(src, dst, mat, q, n);
@@ -184,11 +194,11 @@ __device__ __forceinline__ void wg_gate_k(const T *src, T *dst, const T *__restr
       return wg_tile_gate<T, KL, 2>(src, dst, mat, q, n);
     break;
   case 3:
-    if constexpr (KL >= 3)
+    if constexpr (KL == 3 || KL == 4)
       return wg_tile_gate<T, KL, 3>(src, dst, mat, q, n);
     break;
   case 4:
-    if constexpr (KL >= 4)
+    if constexpr (KL == 4)
       return wg_tile_gate<T, KL, 4>(src, dst, mat, q, n);
     break;
   default:
@@ -337,15 +347,15 @@ __global__ void __launch_bounds__(kMaxBlock, OCC)
       const uint32_t k = __builtin_amdgcn_readfirstlane(d.k);
       const uint32_t moff = __builtin_amdgcn_readfirstlane(d.matrix_offset);
       uint32_t q[kMaxTrajGateQubits];
-      bool ok = k >= 1 && k <= kMaxTrajGateQubits;
+      bool ok = k >= 1 && k <= (sizeof(T) == 4 ? kMaxTrajGateQubits : 4u);
 #pragma unroll
       for (uint32_t b = 0; b < kMaxTrajGateQubits; ++b) {
         q[b] = __builtin_amdgcn_readfirstlane(d.q[b]);
         if (b < k)
           ok = ok && q[b] < n && (b == 0 || q[b] > q[b - 1]);
       }
-      // k = 4 entries hold three real K x K matrices: 3 K^2 / 2 complex slots.
-      const uint64_t slots = k == 4 ? 384 : (uint64_t(1) << (2 * k));
+      // k = 4, 5 entries hold three real K x K matrices: 3 K^2 / 2 complex slots.
+      const uint64_t slots = k >= 4 ? uint64_t(3) << (2 * k - 1) : (uint64_t(1) << (2 * k));
       ok = ok && uint64_t(moff) + slots <= a.n_matrix_elems;
       if (!ok) {
         atomicOr(&work[1], 1u);
@@ -362,8 +372,12 @@ __global__ void __launch_bounds__(kMaxBlock, OCC)
       case 3:
         wg_gate_k<T, 3>(cur, slot, mat, q, n);
         break;
-      default:
+      case 4:
         wg_gate_k<T, 4>(cur, slot, mat, q, n);
+        break;
+      default: // k = 5, f32 only (checked above)
+        if constexpr (sizeof(T) == 4)
+          wg_gate_k<T, 5>(cur, slot, mat, q, n);
         break;
       }
       cur = slot;

@@ -160,8 +160,9 @@ const TrajectoryEngine::Segment &TrajectoryEngine::segment(uint32_t kmax, uint32
   optimize(cg, FusionConfig::size_only(kmax));
   Segment seg;
   for (const GatePtr &g : cg.gates_in_row_order()) {
-    if (g->n_qubits() > 4)
-      throw Error("trajectory fusion produced a gate on more than 4 qubits");
+    if (g->n_qubits() > kmax && g->n_qubits() > 2)
+      throw Error("trajectory fusion produced a gate on more than " + std::to_string(kmax) +
+                  " qubits");
     FusedGate f;
     f.qubits = g->qubits();
     f.m = g->matrix();
@@ -243,8 +244,9 @@ TrajectoryPlan TrajectoryEngine::plan(const std::vector<TrajectorySample> &sampl
   const uint32_t B = tn.chunk_layers, kmax = tn.fusion_max, D = circ_.n_layers();
   if (B < 1 || B > D)
     throw Error("chunk_layers must be in 1..circuit layers");
-  if (kmax < 1 || kmax > 4)
-    throw Error("fusion_max must be in 1..4");
+  const uint32_t kcap = prec_ == CAST_PRECISION_F32 ? 5 : 4; // kernel's gate sizes
+  if (kmax < 1 || kmax > kcap)
+    throw Error("fusion_max must be in 1.." + std::to_string(kcap) + " at this precision");
   if (samples.empty())
     throw Error("no trajectories to run");
   TrajectoryPlan P;
@@ -252,13 +254,13 @@ TrajectoryPlan TrajectoryEngine::plan(const std::vector<TrajectorySample> &sampl
   auto &tasks = P.tasks;
   auto &pool = P.pool;
   std::map<const Segment *, uint32_t> seg_pool; // clean matrices of a segment in this plan
-  // k <= 3: interleaved complex; k = 4: three real matrices Re, Im, Re + Im
-  // (the kernel's three-product form); both in whole complex slots.
+  // k <= 3: interleaved complex; k = 4, 5: three real matrices Re, Im,
+  // Re + Im (the kernel's three-product form); both in whole complex slots.
   auto push_matrix = [&](const ComplexSquareMatrix &m) {
     const uint64_t off = pool.size() / 2;
     if (off + m.data().size() * 2 > 0xffffffffull)
       throw Error("trajectory matrix pool exceeds 2^32 complex entries");
-    if (m.edge_size() == 16) {
+    if (m.edge_size() >= 16) {
       for (const cplx &z : m.data())
         pool.push_back(z.real());
       for (const cplx &z : m.data())
@@ -274,7 +276,7 @@ TrajectoryPlan TrajectoryEngine::plan(const std::vector<TrajectorySample> &sampl
     return uint32_t(off);
   };
   auto slots_of = [](const ComplexSquareMatrix &m) {
-    return uint32_t(m.edge_size() == 16 ? 384 : m.data().size());
+    return uint32_t(m.edge_size() >= 16 ? 3 * m.data().size() / 2 : m.data().size());
   };
   auto seg_offsets = [&](const Segment &seg) {
     auto it = seg_pool.find(&seg);
@@ -595,8 +597,10 @@ TrajectoryTuning TrajectoryEngine::autotune(const std::vector<TrajectorySample> 
       trial(t);
     }
   const TrajectoryTuning shape = best;
-  for (uint32_t kmax : {3u, 4u})
+  for (uint32_t kmax : {3u, 4u, 5u})
     for (uint32_t B : {1u, 2u, 4u, 8u}) {
+      if (kmax == 5 && prec_ != CAST_PRECISION_F32)
+        continue;
       if (B > circ_.n_layers())
         continue;
       TrajectoryTuning t = shape;

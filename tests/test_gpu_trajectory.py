@@ -245,3 +245,31 @@ def test_autotuner_returns_a_tried_tuning_that_matches_oracle():
     assert repr(best) in {repr(t) for t, _ in trials}
     assert all(t.waves_per_simd == 2 and t.slots_per_cu * t.block <= 512 for t, _ in trials)
     check_run(lc, S, F32, best, pairs)
+
+
+@pytest.mark.parametrize("n", [9, 11, 13, 16])
+def test_k5_gates_at_f32_match_oracle(n):
+    """Fusion up to 5 qubits (f32 runs): one-combo-per-thread (n < 11), wave
+    tiles with at most two targets below bit 6, and the one-combo path for
+    k = 5 gates with more."""
+    D = 8
+    lc = cs.LayeredCircuit.random_layers(n, D, 700 + n)
+    S = cs.sample_trajectories(lc, cs.NoiseSpec(0.05, 1, D, 71 * n), 24)
+    pairs = [(i, i + 1) for i in range(n - 1)]
+    tn = cs.TrajectoryTuning(4, 5, 256, 2, 0)
+    eng = cs.TrajectoryEngine(lc, F32, [q for p in pairs for q in p])
+    P = eng.plan(S, tn)
+    k5 = [q for _, q in P["gates"] if len(q) == 5]
+    assert k5, "no 5-qubit fused gates to test"
+    if n == 13:
+        assert any(sum(b < 6 for b in q) > 2 for q in k5)   # the one-combo path
+        assert any(sum(b < 6 for b in q) <= 2 for q in k5)  # the wave-tile path
+    init = np.asarray(cs.random_statevector(n, n + 5))
+    check_run(lc, S, F32, tn, pairs, init)
+
+
+def test_k5_rejected_at_f64_by_planner_and_kernel():
+    lc = cs.LayeredCircuit.random_layers(8, 4, 3)
+    S = cs.sample_trajectories(lc, cs.NoiseSpec(0.05, 1, 4, 4), 4)
+    with pytest.raises(cs.CastError, match="1..4"):
+        cs.TrajectoryEngine(lc, F64, [0, 1]).run(S, cs.TrajectoryTuning(2, 5, 256, 2, 0))

@@ -118,14 +118,16 @@ def test_origin_concatenates_in_application_order():
 
 # ── planner vs oracle ───────────────────────────────────────────────────────
 
-@pytest.mark.parametrize("B,kmax", [(1, 2), (3, 3), (4, 4), (9, 4), (2, 1)])
+@pytest.mark.parametrize("B,kmax", [(1, 2), (3, 3), (4, 4), (9, 4), (2, 1), (4, 5), (9, 5)])
 def test_plan_task_gate_lists_match_explicit_trajectories(B, kmax):
     n, D = 7, 9
     lc = cs.LayeredCircuit.random_layers(n, D, 21)
     S = cs.sample_trajectories(lc, cs.NoiseSpec(0.04, 1, D, 1234 + B), 40)
     assert sum(len(s.errors) for s in S) > 20
     init = np.asarray(cs.random_statevector(n, 8))
-    eng = cs.TrajectoryEngine(lc, cs.Precision.F64, [0, 1])
+    # k = 5 gates exist for f32 runs only; the host plan holds f64 matrices either way
+    prec = cs.Precision.F32 if kmax == 5 else cs.Precision.F64
+    eng = cs.TrajectoryEngine(lc, prec, [0, 1])
     eng.set_initial_state(list(init))
     P = eng.plan(S, cs.TrajectoryTuning(B, kmax, 256, 2, 0))
     pool = P["pool"]
@@ -139,7 +141,7 @@ def test_plan_task_gate_lists_match_explicit_trajectories(B, kmax):
         for off, q in P["gates"][first:first + cnt]:
             assert len(q) <= max(kmax, 2) and list(q) == sorted(q)  # CZ stays 2-qubit
             K = 1 << len(q)
-            if K == 16:  # three real matrices: Re, Im, Re + Im
+            if K >= 16:  # three real matrices: Re, Im, Re + Im
                 reals = pool.view(np.float64)[2 * off:2 * off + 3 * K * K]
                 re, im, rs = reals[:K * K], reals[K * K:2 * K * K], reals[2 * K * K:]
                 assert np.array_equal(rs, re + im)
@@ -188,6 +190,10 @@ def test_plan_rejects_bad_tuning_and_samples():
         eng.plan([cs.TrajectorySample(7, [])], cs.TrajectoryTuning(1, 4))
     with pytest.raises(cs.CastError):
         cs.TrajectoryEngine(lc, cs.Precision.F64, [0, 0])
+    e32 = cs.TrajectoryEngine(lc, cs.Precision.F32, [0, 1])
+    e32.plan(S, cs.TrajectoryTuning(2, 5, 256, 2, 0))
+    with pytest.raises(cs.CastError, match="1..5"):
+        e32.plan(S, cs.TrajectoryTuning(2, 6, 256, 2, 0))
     with pytest.raises(cs.CastError):
         cs.TrajectoryEngine(lc, cs.Precision.F64, [0, 5])
 
