@@ -299,5 +299,23 @@ __host__ __device__ inline TileLayout tile_layout(const uint32_t *q, uint32_t k)
   return t;
 }
 
+// v_mfma_{f32,f64}_16x16x4: lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16]; accumulator register i of lane l holds
+// D[4 (l / 16) + i][l % 16] (f32) or D[l / 16 + 4 i][l % 16] (f64).
+template <typename T> struct Acc4;
+template <> struct Acc4<float> {
+  using type = float __attribute__((ext_vector_type(4)));
+};
+template <> struct Acc4<double> {
+  using type = double __attribute__((ext_vector_type(4)));
+};
+
+__device__ __forceinline__ Acc4<float>::type mfma16(float a, float b, Acc4<float>::type c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ Acc4<double>::type mfma16(double a, double b, Acc4<double>::type c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
 } // namespace dev
 } // namespace cast_hip

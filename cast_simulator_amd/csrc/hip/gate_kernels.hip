@@ -184,20 +184,8 @@ __global__ void __launch_bounds__(256) k_monomial(T *__restrict__ sv, MonoArgs<T
 // matrices are loaded into registers once per wave and reused for every tile
 // the wave processes.
 
-template <typename T> struct Acc4;
-template <> struct Acc4<float> {
-  using type = float __attribute__((ext_vector_type(4)));
-};
-template <> struct Acc4<double> {
-  using type = double __attribute__((ext_vector_type(4)));
-};
-
-__device__ __forceinline__ Acc4<float>::type mfma16(float a, float b, Acc4<float>::type c) {
-  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-}
-__device__ __forceinline__ Acc4<double>::type mfma16(double a, double b, Acc4<double>::type c) {
-  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
+using dev::Acc4;
+using dev::mfma16;
 
 template <typename T, int KL>
 __global__ void __launch_bounds__(256) k_mfma(T *__restrict__ sv, const T *__restrict__ m3,

@@ -165,6 +165,82 @@ __device__ __forceinline__ void wg_dense_gate(const T *src, T *dst, const T *__r
   }
 }
 
+// k = 5 at f32 on the matrix cores: the three-product contraction of the
+// plain-path matrix-core kernel (Out = M V over tiles of 16 combos with
+// v_mfma_f32_16x16x4), run by the workgroup's waves over the statevector.
+// Lane l owns combo l % 16 of a tile and loads inputs 4 s + l / 16; the
+// three real K x K matrices ([Re | Im | Re + Im] from the pool) stay in
+// registers for the whole gate.  Two tiles per step keep 16 loads per lane
+// in flight.  Any target set: addresses come from the combo's bit deposit.
+// (The VALU three-product update of k = 5 costs 12 flops per byte at f32,
+// above the 9.8 flop/B VALU ridge of the chip; here it is MFMA work.)
+template <int KL>
+__device__ __forceinline__ void wg_mfma_gate(const float *src, float *dst,
+                                             const float *__restrict__ m3, const uint32_t *q,
+                                             uint32_t n) {
+  constexpr int K = 1 << KL, RB = K / 16, KS = K / 4, U = 2;
+  using VT = typename V2<float>::type;
+  using AV = dev::Acc4<float>::type;
+  const VT *in = reinterpret_cast<const VT *>(src);
+  VT *out = reinterpret_cast<VT *>(dst);
+  const uint32_t lane = threadIdx.x & 63u, j = lane >> 4, col = lane & 15u;
+  const uint32_t wave = threadIdx.x >> 6, n_waves = blockDim.x >> 6;
+  float a[3][RB][KS];
+#pragma unroll
+  for (int m = 0; m < 3; ++m)
+#pragma unroll
+    for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+      for (int s = 0; s < KS; ++s)
+        a[m][rb][s] = m3[m * K * K + (16 * rb + col) * K + 4 * s + j];
+  uint64_t din[KS];
+#pragma unroll
+  for (int s = 0; s < KS; ++s)
+    din[s] = local_delta<KL>(4 * s + j, q);
+  // accumulator register i of lane l holds output row 16 rb + 4 (l / 16) + i
+  uint64_t dout[RB][4];
+#pragma unroll
+  for (int rb = 0; rb < RB; ++rb)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      dout[rb][i] = local_delta<KL>(16 * rb + 4 * j + i, q);
+  // n >= KL + 5, so n_tiles = 2^(n - KL - 4) is a multiple of U
+  const uint64_t n_tiles = uint64_t(1) << (n - KL - 4);
+  for (uint64_t t0 = uint64_t(wave) * U; t0 < n_tiles; t0 += uint64_t(n_waves) * U) {
+    uint64_t base[U];
+    float br[U][KS], bi[U][KS];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      base[u] = combo_base<KL>((t0 + u) * 16 + col, q);
+#pragma unroll
+      for (int s = 0; s < KS; ++s) {
+        const VT x = in[base[u] + din[s]];
+        br[u][s] = x.x;
+        bi[u][s] = x.y;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int rb = 0; rb < RB; ++rb) {
+        AV t1 = {0, 0, 0, 0}, t2 = {0, 0, 0, 0}, t3 = {0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+          t1 = dev::mfma16(a[0][rb][s], br[u][s], t1);
+          t2 = dev::mfma16(a[1][rb][s], bi[u][s], t2);
+          t3 = dev::mfma16(a[2][rb][s], br[u][s] + bi[u][s], t3);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          VT o;
+          o.x = t1[i] - t2[i];
+          o.y = t3[i] - t1[i] - t2[i];
+          out[base[u] + dout[rb][i]] = o;
+        }
+      }
+  }
+}
+
 template <typename T, int KL>
 __device__ __forceinline__ void wg_gate_k(const T *src, T *dst, const T *__restrict__ mat,
                                           const uint32_t *q, uint32_t n) {
@@ -174,35 +250,32 @@ __device__ __forceinline__ void wg_gate_k(const T *src, T *dst, const T *__restr
 #pragma unroll
   for (int b = 0; b < KL; ++b)
     n_low += q[b] < 6;
-  if constexpr (KL == 5) {
-    // Exchanging three or more target bits across lanes for K = 32 needs
-    // more registers than the kernel's budget (it would spill in every
-    // path); such gates, rare at this size, take the one-combo-per-thread
-    // path.
-    if (n_low > 2)
-      return wg_dense_gate<T, KL>(src, dst, mat, q, n);
-  }
-  switch (n_low) {
-  case 0:
-    return wg_tile_gate<T, KL, 0>(src, dst, mat, q, n);
-  case 1:
-    if constexpr (KL >= 1)
-      return wg_tile_gate<T, KL, 1>(src, dst, mat, q, n);
-    break;
-  case 2:
-    if constexpr (KL >= 2)
-      return wg_tile_gate<T, KL, 2>(src, dst, mat, q, n);
-    break;
-  case 3:
-    if constexpr (KL == 3 || KL == 4)
-      return wg_tile_gate<T, KL, 3>(src, dst, mat, q, n);
-    break;
-  case 4:
-    if constexpr (KL == 4)
-      return wg_tile_gate<T, KL, 4>(src, dst, mat, q, n);
-    break;
-  default:
-    break;
+  if constexpr (KL == 5) { // f32 only: matrix cores, any targets
+    if constexpr (sizeof(T) == 4)
+      wg_mfma_gate<KL>(src, dst, mat, q, n);
+  } else {
+    switch (n_low) {
+    case 0:
+      return wg_tile_gate<T, KL, 0>(src, dst, mat, q, n);
+    case 1:
+      if constexpr (KL >= 1)
+        return wg_tile_gate<T, KL, 1>(src, dst, mat, q, n);
+      break;
+    case 2:
+      if constexpr (KL >= 2)
+        return wg_tile_gate<T, KL, 2>(src, dst, mat, q, n);
+      break;
+    case 3:
+      if constexpr (KL == 3 || KL == 4)
+        return wg_tile_gate<T, KL, 3>(src, dst, mat, q, n);
+      break;
+    case 4:
+      if constexpr (KL == 4)
+        return wg_tile_gate<T, KL, 4>(src, dst, mat, q, n);
+      break;
+    default:
+      break;
+    }
   }
 }
 
