@@ -439,7 +439,7 @@ def run_serving(cs, count: int, rate: float, capacity_sample: int):
     return {
         "workload": f"{count} requests, Poisson arrivals at {rate}/s (seed {SEED}), "
                     "random-layer circuits n ~ U{12..28}, depth ~ U{10..50}, f32/f64 50/50, "
-                    "fused to k<=5, resident statevectors, admission by free HBM",
+                    "fused to k<=6, resident statevectors, admission by free HBM",
         "circuits_per_s": round(st["throughput"], 2),
         "p50_ms": round(st["p50_s"] * 1e3, 2),
         "p99_ms": round(st["p99_s"] * 1e3, 2),

@@ -44,7 +44,9 @@ std::vector<ServeRequest> poisson_requests(uint32_t count, double rate, uint32_t
 struct ServeConfig {
   uint32_t streams = 8;          // concurrent requests on the device
   uint32_t compile_threads = 8;  // host threads compiling arrived requests
-  uint32_t fusion_max = 5;       // size-only fusion bound (1..6)
+  uint32_t fusion_max = 6;       // size-only fusion bound (1..6); k = 6 runs on
+                                 // matrix cores and fewer passes serve more
+                                 // circuits per second than k <= 5
   uint64_t budget_bytes = 0;     // statevector memory; 0 = free memory - reserve
   uint64_t reserve_bytes = uint64_t(8) << 30;
   bool realtime = true;          // release requests at their arrival times
