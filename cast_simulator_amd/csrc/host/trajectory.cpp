@@ -597,10 +597,11 @@ TrajectoryTuning TrajectoryEngine::autotune(const std::vector<TrajectorySample> 
       trial(t);
     }
   const TrajectoryTuning shape = best;
-  for (uint32_t kmax : {3u, 4u, 5u})
+  // Fusion sizes: the two largest the kernel takes at this precision (3
+  // never won a measurement against 4).
+  const bool f32 = prec_ == CAST_PRECISION_F32;
+  for (uint32_t kmax : {f32 ? 4u : 3u, f32 ? 5u : 4u})
     for (uint32_t B : {1u, 2u, 4u, 8u}) {
-      if (kmax == 5 && prec_ != CAST_PRECISION_F32)
-        continue;
       if (B > circ_.n_layers())
         continue;
       TrajectoryTuning t = shape;
