@@ -291,27 +291,35 @@ __global__ void __launch_bounds__(kMfma6Block) k_mfma6(T *__restrict__ sv, const
       bi[s] = x.y;
       bs[s] = x.x + x.y;
     }
+    // Two 16-row output blocks at a time: six independent accumulator
+    // chains keep the matrix cores busy between dependent MFMAs.
 #pragma unroll
-    for (int rb = 0; rb < RB; ++rb) {
+    for (int rp = 0; rp < RB; rp += 2) {
       // Re-read per tile (not hoisted out of the tile loop into 128+ VGPRs).
-      const T *a = a_lds + rb * KS * 64 + lane + dev::opaque_u32(0);
-      AV t1 = {0, 0, 0, 0}, t2 = {0, 0, 0, 0}, t3 = {0, 0, 0, 0};
+      const T *a = a_lds + rp * KS * 64 + lane + dev::opaque_u32(0);
+      AV t1[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}}, t2[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}},
+         t3[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
 #pragma unroll
       for (int s = 0; s < KS; ++s) {
-        const T ar = a[s * 64];
-        const T ai = a[RB * KS * 64 + s * 64];
-        t1 = mfma16(ar, br[s], t1);
-        t2 = mfma16(ai, bi[s], t2);
-        t3 = mfma16(ar + ai, bs[s], t3);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const T ar = a[h * KS * 64 + s * 64];
+          const T ai = a[RB * KS * 64 + h * KS * 64 + s * 64];
+          t1[h] = mfma16(ar, br[s], t1[h]);
+          t2[h] = mfma16(ai, bi[s], t2[h]);
+          t3[h] = mfma16(ar + ai, bs[s], t3[h]);
+        }
       }
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        VT o;
-        o.x = t1[i] - t2[i];
-        o.y = t3[i] - t1[i] - t2[i];
-        const uint32_t row = 16 * rb + (sizeof(T) == 8 ? j + 4 * i : 4 * j + i);
-        amp[base + local_delta<KL>(row, g.q)] = o;
-      }
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          VT o;
+          o.x = t1[h][i] - t2[h][i];
+          o.y = t3[h][i] - t1[h][i] - t2[h][i];
+          const uint32_t row = 16 * (rp + h) + (sizeof(T) == 8 ? j + 4 * i : 4 * j + i);
+          amp[base + local_delta<KL>(row, g.q)] = o;
+        }
     }
   }
 }
