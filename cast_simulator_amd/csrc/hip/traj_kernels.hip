@@ -7,11 +7,11 @@
 // every later gate updates the slot in place, with a workgroup barrier
 // between gates.  Gates use wave tiles (coalesced 64-amplitude rows, cross-
 // lane exchange of targets below bit 6) when n >= k + 6, else one combo per
-// thread.  A wave keeps at least 16 amplitude loads in flight per lane
-// whatever k is (16 / K tiles per step for k < 4, one tile of K for k = 4, 5),
-// since one workgroup alone must cover the memory latency of its
-// statevector.  Gates of k = 4 and 5 (k = 5 in f32 only) use the
-// three-product form: the pool holds [Re M | Im M | Re M + Im M].  At the end the workgroup reduces, in f64, the
+// thread.  A wave keeps 16 amplitude loads in flight per lane whatever k is
+// (16 / K tiles per step for k <= 4), since one workgroup alone must cover
+// the memory latency of its statevector.  Gates of k = 4 and 5 (k = 5 in f32
+// only) use the three-product form, the pool holding [Re M | Im M | Re M +
+// Im M]: k = 4 on the VALU, k = 5 on the matrix cores (wg_mfma_gate).  At the end the workgroup reduces, in f64, the
 // squared norm and sum |amp|^2 (-1)^(bit a xor bit b) for each requested
 // pair (a, b), and can copy the final state out.
 //
