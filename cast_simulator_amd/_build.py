@@ -74,8 +74,12 @@ def build_host() -> Path:
     import pybind11
 
     py_inc = sysconfig.get_paths()["include"]
+    # -fcx-limited-range: complex products as (ac - bd, ad + bc), the
+    # reference's num-complex formula, instead of a call to the C99 Annex G
+    # routine (same values for finite operands, several times faster).
     flags = [
-        "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fvisibility=hidden",
+        "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fcx-limited-range",
+        "-fvisibility=hidden",
         f"-I{INCLUDE}", f"-I{pybind11.get_include()}", f"-I{py_inc}",
     ]
     srcs = sorted((PKG / "csrc" / "host").glob("*.cpp"))

@@ -213,13 +213,18 @@ ComplexSquareMatrix ComplexSquareMatrix::random_sparse(size_t edge, double spars
 ComplexSquareMatrix ComplexSquareMatrix::matmul(const ComplexSquareMatrix &o) const {
   if (n_ != o.n_)
     throw Error("matrix size mismatch");
+  // Row r of the product accumulates its terms in k order (as a dot
+  // product would), with rows of `o` read contiguously.
   ComplexSquareMatrix out(n_);
+  const cplx *a = d_.data(), *b = o.d_.data();
+  cplx *d = out.d_.data();
   for (size_t r = 0; r < n_; ++r)
-    for (size_t c = 0; c < n_; ++c) {
-      cplx acc(0, 0);
-      for (size_t k = 0; k < n_; ++k)
-        acc += get(r, k) * o.get(k, c);
-      out.set(r, c, acc);
+    for (size_t k = 0; k < n_; ++k) {
+      const cplx x = a[r * n_ + k];
+      const cplx *brow = b + k * n_;
+      cplx *drow = d + r * n_;
+      for (size_t c = 0; c < n_; ++c)
+        drow[c] += x * brow[c];
     }
   return out;
 }
