@@ -9,9 +9,9 @@
 // lane exchange of targets below bit 6) when n >= k + 6, else one combo per
 // thread.  A wave keeps 16 amplitude loads in flight per lane whatever k is
 // (16 / K tiles per step for k <= 4), since one workgroup alone must cover
-// the memory latency of its statevector.  Gates of k = 4 and 5 (k = 5 in f32
-// only) use the three-product form, the pool holding [Re M | Im M | Re M +
-// Im M]: k = 4 on the VALU, k = 5 on the matrix cores (wg_mfma_gate).  At the end the workgroup reduces, in f64, the
+// the memory latency of its statevector.  Gates of k = 4 to 6 (k = 5, 6 in
+// f32 only) use the three-product form, the pool holding [Re M | Im M | Re M
+// + Im M]: k = 4 on the VALU, k = 5 and 6 on the matrix cores.  At the end the workgroup reduces, in f64, the
 // squared norm and sum |amp|^2 (-1)^(bit a xor bit b) for each requested
 // pair (a, b), and can copy the final state out.
 //
@@ -35,7 +35,7 @@ using dev::V2;
 constexpr int kMaxPairs = 32;
 constexpr int kMaxBlock = 512;
 constexpr int kMaxWaves = kMaxBlock / 64;
-constexpr uint32_t kMaxTrajGateQubits = 5; // 4 at f64
+constexpr uint32_t kMaxTrajGateQubits = 6; // 4 at f64
 
 // Layouts the host bindings rely on (no padding surprises across compilers).
 static_assert(sizeof(cast_hip_traj_gate_t) == 12, "cast_hip_traj_gate_t layout");
@@ -241,6 +241,74 @@ __device__ __forceinline__ void wg_mfma_gate(const float *src, float *dst,
   }
 }
 
+// k = 6 at f32 on the matrix cores, as the plain path's k = 6 kernel: Re M
+// and Im M (32 KiB) are staged in LDS once per gate, laid out so that the
+// operand (m, rb, s) of lane l is element ((m * 4 + rb) * 16 + s) * 64 + l
+// (every read is 64 consecutive values); Re M + Im M is formed on the fly.
+// A lane owns combo l % 16 of a 16-combo tile and keeps its 16 inputs in
+// registers while two 16-row output blocks are produced per pass (six
+// independent accumulator chains).  Needs n >= 12 (checked by the caller).
+// The caller's barrier after every gate keeps the next gate's staging from
+// overwriting operands still in use.
+__device__ __forceinline__ void wg_mfma6_gate(const float *src, float *dst,
+                                              const float *__restrict__ m3, const uint32_t *q,
+                                              uint32_t n) {
+  constexpr int KL = 6, K = 64, RB = 4, KS = 16;
+  using VT = typename V2<float>::type;
+  using AV = dev::Acc4<float>::type;
+  __shared__ float a_lds[2 * RB * KS * 64];
+  for (uint32_t i = threadIdx.x; i < 2 * RB * KS * 64; i += blockDim.x) {
+    const uint32_t l = i & 63u, s = (i >> 6) & (KS - 1), rb = (i >> 10) & (RB - 1), m = i >> 12;
+    a_lds[i] = m3[m * K * K + (16 * rb + (l & 15u)) * K + 4 * s + (l >> 4)];
+  }
+  __syncthreads();
+  const VT *in = reinterpret_cast<const VT *>(src);
+  VT *out = reinterpret_cast<VT *>(dst);
+  const uint32_t lane = threadIdx.x & 63u, j = lane >> 4, col = lane & 15u;
+  const uint32_t wave = threadIdx.x >> 6, n_waves = blockDim.x >> 6;
+  uint64_t din[KS];
+#pragma unroll
+  for (int s = 0; s < KS; ++s)
+    din[s] = local_delta<KL>(4 * s + j, q);
+  const uint64_t n_tiles = uint64_t(1) << (n - KL - 4);
+  for (uint64_t t = wave; t < n_tiles; t += n_waves) {
+    const uint64_t base = combo_base<KL>(t * 16 + col, q);
+    float br[KS], bi[KS];
+#pragma unroll
+    for (int s = 0; s < KS; ++s) {
+      const VT x = in[base + din[s]];
+      br[s] = x.x;
+      bi[s] = x.y;
+    }
+#pragma unroll
+    for (int rp = 0; rp < RB; rp += 2) {
+      // Re-read per tile (not hoisted out of the tile loop into registers).
+      const float *a = a_lds + rp * KS * 64 + lane + opaque_u32(0);
+      AV t1[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}}, t2[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}},
+         t3[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+#pragma unroll
+      for (int s = 0; s < KS; ++s)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const float ar = a[h * KS * 64 + s * 64];
+          const float ai = a[RB * KS * 64 + h * KS * 64 + s * 64];
+          t1[h] = dev::mfma16(ar, br[s], t1[h]);
+          t2[h] = dev::mfma16(ai, bi[s], t2[h]);
+          t3[h] = dev::mfma16(ar + ai, br[s] + bi[s], t3[h]);
+        }
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          VT o;
+          o.x = t1[h][i] - t2[h][i];
+          o.y = t3[h][i] - t1[h][i] - t2[h][i];
+          out[base + local_delta<KL>(16 * (rp + h) + 4 * j + i, q)] = o;
+        }
+    }
+  }
+}
+
 template <typename T, int KL>
 __device__ __forceinline__ void wg_gate_k(const T *src, T *dst, const T *__restrict__ mat,
                                           const uint32_t *q, uint32_t n) {
@@ -421,6 +489,7 @@ __global__ void __launch_bounds__(kMaxBlock, OCC)
       const uint32_t moff = __builtin_amdgcn_readfirstlane(d.matrix_offset);
       uint32_t q[kMaxTrajGateQubits];
       bool ok = k >= 1 && k <= (sizeof(T) == 4 ? kMaxTrajGateQubits : 4u);
+      ok = ok && (k < 6 || n >= 12); // the k = 6 path needs 64 tiles of 16 combos
 #pragma unroll
       for (uint32_t b = 0; b < kMaxTrajGateQubits; ++b) {
         q[b] = __builtin_amdgcn_readfirstlane(d.q[b]);
@@ -448,9 +517,13 @@ __global__ void __launch_bounds__(kMaxBlock, OCC)
       case 4:
         wg_gate_k<T, 4>(cur, slot, mat, q, n);
         break;
-      default: // k = 5, f32 only (checked above)
+      case 5: // f32 only (checked above)
         if constexpr (sizeof(T) == 4)
           wg_gate_k<T, 5>(cur, slot, mat, q, n);
+        break;
+      default: // k = 6, f32 only, n >= 12 (checked above)
+        if constexpr (sizeof(T) == 4)
+          wg_mfma6_gate(cur, slot, mat, q, n);
         break;
       }
       cur = slot;

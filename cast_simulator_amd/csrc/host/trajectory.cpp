@@ -244,9 +244,13 @@ TrajectoryPlan TrajectoryEngine::plan(const std::vector<TrajectorySample> &sampl
   const uint32_t B = tn.chunk_layers, kmax = tn.fusion_max, D = circ_.n_layers();
   if (B < 1 || B > D)
     throw Error("chunk_layers must be in 1..circuit layers");
-  const uint32_t kcap = prec_ == CAST_PRECISION_F32 ? 5 : 4; // kernel's gate sizes
+  // Gate sizes the kernel takes: k <= 4 at f64; k <= 6 at f32, k = 6 from 12
+  // qubits on (its matrix-core path works on 64 tiles of 16 combos or more).
+  const uint32_t kcap =
+      prec_ != CAST_PRECISION_F32 ? 4 : (circ_.n_qubits >= 12 ? 6 : 5);
   if (kmax < 1 || kmax > kcap)
-    throw Error("fusion_max must be in 1.." + std::to_string(kcap) + " at this precision");
+    throw Error("fusion_max must be in 1.." + std::to_string(kcap) +
+                " at this precision and size");
   if (samples.empty())
     throw Error("no trajectories to run");
   TrajectoryPlan P;
@@ -597,10 +601,11 @@ TrajectoryTuning TrajectoryEngine::autotune(const std::vector<TrajectorySample> 
       trial(t);
     }
   const TrajectoryTuning shape = best;
-  // Fusion sizes: the two largest the kernel takes at this precision (3
-  // never won a measurement against 4).
-  const bool f32 = prec_ == CAST_PRECISION_F32;
-  for (uint32_t kmax : {f32 ? 4u : 3u, f32 ? 5u : 4u})
+  // Fusion sizes: the two largest the kernel takes at this precision and
+  // size (smaller ones never won a measurement).
+  const uint32_t ktop =
+      prec_ != CAST_PRECISION_F32 ? 4u : (circ_.n_qubits >= 12 ? 6u : 5u);
+  for (uint32_t kmax : {ktop - 1, ktop})
     for (uint32_t B : {1u, 2u, 4u, 8u}) {
       if (B > circ_.n_layers())
         continue;

@@ -291,16 +291,17 @@ int cast_hip_inner_product(uint64_t a, uint8_t precision_a, uint64_t b, uint8_t 
  * on the host, src/simulator/trajectory.rs:180-265).  Each workgroup owns
  * one statevector slot and takes whole tasks from an atomic counter until the
  * task list is exhausted (no workgroup ever waits on another).  A task starts
- * from a checkpoint statevector, applies its gate list (k <= 4 at F64, <= 5
- * at F32) — the first gate reads the checkpoint and writes the slot — and
- * writes f64 sums of |amp|^2 and of |amp|^2 (-1)^(bit a xor bit b) for each
- * requested pair.  Matrix pool entries (run precision): k = 1..3, the K x K
- * complex matrix, row-major interleaved (K^2 complex slots); k = 4, 5, three
- * real K x K matrices Re M, Im M, Re M + Im M one after the other
- * (3 K^2 / 2 complex slots), for the three-product update. */
+ * from a checkpoint statevector, applies its gate list (k <= 4 at F64, <= 6
+ * at F32, k = 6 only on statevectors of >= 12 qubits) — the first gate reads
+ * the checkpoint and writes the slot — and writes f64 sums of |amp|^2 and of
+ * |amp|^2 (-1)^(bit a xor bit b) for each requested pair.  Matrix pool
+ * entries (run precision): k = 1..3, the K x K complex matrix, row-major
+ * interleaved (K^2 complex slots); k = 4..6, three real K x K matrices Re M,
+ * Im M, Re M + Im M one after the other (3 K^2 / 2 complex slots), for the
+ * three-product update. */
 typedef struct cast_hip_traj_gate_t {
   uint32_t matrix_offset; /* complex elements into the matrix pool      */
-  uint8_t k;              /* 1..4 (F64), 1..5 (F32)                     */
+  uint8_t k;              /* 1..4 (F64), 1..6 (F32)                     */
   uint8_t q[6];           /* ascending target qubits                    */
   uint8_t pad;
 } cast_hip_traj_gate_t;

@@ -273,3 +273,26 @@ def test_k5_rejected_at_f64_by_planner_and_kernel():
     S = cs.sample_trajectories(lc, cs.NoiseSpec(0.05, 1, 4, 4), 4)
     with pytest.raises(cs.CastError, match="1..4"):
         cs.TrajectoryEngine(lc, F64, [0, 1]).run(S, cs.TrajectoryTuning(2, 5, 256, 2, 0))
+
+
+@pytest.mark.parametrize("n", [12, 13, 16])
+def test_k6_gates_at_f32_match_oracle(n):
+    """Fusion up to 6 qubits (f32 runs, n >= 12): the LDS-staged matrix-core
+    path, alongside k <= 5 gates of every kind."""
+    D = 8
+    lc = cs.LayeredCircuit.random_layers(n, D, 900 + n)
+    S = cs.sample_trajectories(lc, cs.NoiseSpec(0.05, 1, D, 37 * n), 24)
+    pairs = [(i, i + 1) for i in range(n - 1)]
+    tn = cs.TrajectoryTuning(8, 6, 256, 2, 0)
+    eng = cs.TrajectoryEngine(lc, F32, [q for p in pairs for q in p])
+    P = eng.plan(S, tn)
+    assert any(len(q) == 6 for _, q in P["gates"]), "no 6-qubit fused gates to test"
+    init = np.asarray(cs.random_statevector(n, n + 9))
+    check_run(lc, S, F32, tn, pairs, init)
+
+
+def test_k6_needs_twelve_qubits():
+    lc = cs.LayeredCircuit.random_layers(11, 4, 3)
+    S = cs.sample_trajectories(lc, cs.NoiseSpec(0.05, 1, 4, 4), 4)
+    with pytest.raises(cs.CastError, match="1..5"):
+        cs.TrajectoryEngine(lc, F32, [0, 1]).run(S, cs.TrajectoryTuning(2, 6, 256, 2, 0))
