@@ -333,7 +333,7 @@ def run_trajectories(cs, n: int, count: int, tune_sample: int, plain_sample: int
     n_err = sum(len(s.errors) for s in S)
     eng = cs.TrajectoryEngine(lc, cs.Precision.F32, pairs)
     t0 = time.perf_counter()
-    best, trials = eng.autotune(S[:tune_sample], cs.TrajectoryTuning(2, 4, 256, 2, 0))
+    best, trials = eng.autotune(S[:tune_sample], cs.TrajectoryTuning(4, 6, 256, 2, 0))
     t_tune = time.perf_counter() - t0
     log(f"  trajectory autotune: {len(trials)} trials on {tune_sample} trajectories, "
         f"{t_tune:.1f} s -> {best!r}")
