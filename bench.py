@@ -536,7 +536,7 @@ def main() -> int:
                     help="dense-large: the 33q f64 / 34q f32 k<=6 matrix-core workload; "
                          "trajectories: the batched-trajectory launch alone with a fixed "
                          "tuning (for profiling); each prints its own JSON line")
-    ap.add_argument("--traj-tuning", default="4,6,256,2,2",
+    ap.add_argument("--traj-tuning", default="4,6,512,1,2",
                     help="chunk_layers,fusion_max,block,slots_per_cu,waves_per_simd for "
                          "--workload trajectories")
     ap.add_argument("--no-traj", action="store_true")

@@ -3,7 +3,7 @@
     python -m cast_simulator_amd.rocprof_summary kernel_trace.csv [--last N]
     python -m cast_simulator_amd.rocprof_summary counter_collection.csv --counters
     python -m cast_simulator_amd.rocprof_summary FETCH.csv --traffic WRITE.csv \
-        --copy-bytes B --last N --algorithmic A
+        --copy-bytes B --last N --algorithmic A [--kernels k_traj]
 
 Kernel-trace mode prints per-kernel dispatch statistics (count, mean, min,
 max, total) over all dispatches, and, with --last N, over the last N
@@ -16,6 +16,8 @@ dispatch) of the same program into HBM bytes per launch of each gate kernel
 over the last N gate dispatches, calibrated by what the same counters read
 for the device-to-device copy of B bytes the program makes first (the
 `--copy-probe-qubits` option of bench.py): factor = counter bytes / B.
+--kernels replaces the gate-kernel name prefixes (k_tile, k_mfma, k_dense,
+k_monomial) with others, e.g. k_traj for the batched-trajectory launch.
 """
 from __future__ import annotations
 
@@ -79,21 +81,22 @@ def counters(path: str) -> str:
     return "\n".join(out) + "\n"
 
 
-def _gate_and_copy(path: str):
+def _gate_and_copy(path: str, prefixes=GATE_KERNELS):
     with open(path, newline="") as f:
         rows = list(csv.DictReader(f))
     rows.sort(key=lambda r: int(r["Dispatch_Id"]))
     copy = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"]]
     gate = [(_short(r["Kernel_Name"]), float(r["Counter_Value"])) for r in rows
-            if _short(r["Kernel_Name"]).startswith(GATE_KERNELS)]
+            if _short(r["Kernel_Name"]).startswith(prefixes)]
     return max(copy) if copy else None, gate
 
 
-def traffic(fetch_csv: str, write_csv: str, copy_bytes: int, last: int, algorithmic: float):
+def traffic(fetch_csv: str, write_csv: str, copy_bytes: int, last: int, algorithmic: float,
+            prefixes=GATE_KERNELS):
     import json
 
-    fcopy, fgate = _gate_and_copy(fetch_csv)
-    wcopy, wgate = _gate_and_copy(write_csv)
+    fcopy, fgate = _gate_and_copy(fetch_csv, prefixes)
+    wcopy, wgate = _gate_and_copy(write_csv, prefixes)
     if not fcopy or not wcopy:
         raise SystemExit("no copy-probe dispatch found in the counter files")
     ff, wf = fcopy * 1024 / copy_bytes, wcopy * 1024 / copy_bytes
@@ -129,9 +132,12 @@ def main() -> None:
     ap.add_argument("--traffic", metavar="WRITE_CSV", default=None)
     ap.add_argument("--copy-bytes", type=int, default=0)
     ap.add_argument("--algorithmic", type=float, default=0.0)
+    ap.add_argument("--kernels", nargs="+", default=None,
+                    help="kernel-name prefixes for traffic mode (default: the gate kernels)")
     a = ap.parse_args()
     if a.traffic:
-        print(traffic(a.csv, a.traffic, a.copy_bytes, a.last or 0, a.algorithmic))
+        print(traffic(a.csv, a.traffic, a.copy_bytes, a.last or 0, a.algorithmic,
+                      tuple(a.kernels) if a.kernels else GATE_KERNELS))
     else:
         print(counters(a.csv) if a.counters else kernel_trace(a.csv, a.last))
 
