@@ -33,9 +33,11 @@ namespace {
 
 using dev::combo_base;
 using dev::insert_zero;
+using dev::ld_amp;
 using dev::ld_stream;
 using dev::local_delta;
 using dev::NV;
+using dev::st_amp;
 using dev::st_stream;
 using dev::V2;
 
@@ -157,14 +159,14 @@ __global__ void __launch_bounds__(256) k_monomial(T *__restrict__ sv, MonoArgs<T
 #pragma unroll
     for (int j = 0; j < K; ++j)
       if (uint32_t(j) < m.n_active)
-        x[j] = amp[base + local_delta<KL>(m.col[j], g.q)];
+        x[j] = ld_amp(amp + base + local_delta<KL>(m.col[j], g.q));
 #pragma unroll
     for (int j = 0; j < K; ++j)
       if (uint32_t(j) < m.n_active) {
         VT o;
         o.x = m.re[j] * x[j].x - m.im[j] * x[j].y;
         o.y = m.re[j] * x[j].y + m.im[j] * x[j].x;
-        amp[base + local_delta<KL>(m.row[j], g.q)] = o;
+        st_amp(amp + base + local_delta<KL>(m.row[j], g.q), o);
       }
   }
 }
@@ -224,7 +226,7 @@ __global__ void __launch_bounds__(256) k_mfma(T *__restrict__ sv, const T *__res
     T br[KS], bi[KS], bs[KS];
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
-      const VT x = amp[base + din[s]];
+      const VT x = ld_amp(amp + base + din[s]);
       br[s] = x.x;
       bi[s] = x.y;
       bs[s] = x.x + x.y;
@@ -243,7 +245,7 @@ __global__ void __launch_bounds__(256) k_mfma(T *__restrict__ sv, const T *__res
         VT o;
         o.x = t1[i] - t2[i];
         o.y = t3[i] - t1[i] - t2[i];
-        amp[base + dout[rb][i]] = o;
+        st_amp(amp + base + dout[rb][i], o);
       }
     }
   }
@@ -286,7 +288,7 @@ __global__ void __launch_bounds__(kMfma6Block) k_mfma6(T *__restrict__ sv, const
     T br[KS], bi[KS], bs[KS];
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
-      const VT x = amp[base + din[s]];
+      const VT x = ld_amp(amp + base + din[s]);
       br[s] = x.x;
       bi[s] = x.y;
       bs[s] = x.x + x.y;
@@ -318,7 +320,7 @@ __global__ void __launch_bounds__(kMfma6Block) k_mfma6(T *__restrict__ sv, const
           o.x = t1[h][i] - t2[h][i];
           o.y = t3[h][i] - t1[h][i] - t2[h][i];
           const uint32_t row = 16 * (rp + h) + (sizeof(T) == 8 ? j + 4 * i : 4 * j + i);
-          amp[base + local_delta<KL>(row, g.q)] = o;
+          st_amp(amp + base + local_delta<KL>(row, g.q), o);
         }
     }
   }
