@@ -291,14 +291,19 @@ def run_density_matrix(cs, n: int = 15, depth: int = 4, p: float = 1e-3, steps: 
     rho = cs.HipStatevector(2 * n, cs.Precision.F64)
     rho.zero()
     mgr.tune(ids, rho, 16)
+    fam = {i: mgr.family(i, 2 * n) for i in set(ids)}
+    kq = {i: mgr.plan_info(i).k for i in set(ids)}
     rho.zero()
     gpu = 0.0
+    per_kernel = defaultdict(list)
     t0 = time.perf_counter()
     for _ in range(steps):
         for i in ids:
             mgr.apply(i, rho)
         kernels, _ = mgr.sync(True)
         gpu += sum(t for _, t in kernels)
+        for kid, t in kernels:
+            per_kernel[f"{KERNEL[fam[kid]]} k={kq[kid]}"].append(t)
     wall = time.perf_counter() - t0
     nbytes = 2 * (1 << (2 * n)) * 16
     trace = float(abi.dm_populations(rho.dptr(), n, abi.F64).sum())
@@ -311,6 +316,8 @@ def run_density_matrix(cs, n: int = 15, depth: int = 4, p: float = 1e-3, steps: 
             "gates": len(ids), "superop_k_histogram": dict(sorted(hist.items())),
             "steps": steps, "gbs_per_gate": round(len(ids) * steps * nbytes / gpu / 1e9, 1),
             "wall_gbs_per_gate": round(len(ids) * steps * nbytes / wall / 1e9, 1),
+            "per_kernel": {k: {"launches": len(v), "gbs": round(len(v) * nbytes / sum(v) / 1e9, 1)}
+                           for k, v in sorted(per_kernel.items())},
             "trace_after": trace}
 
 

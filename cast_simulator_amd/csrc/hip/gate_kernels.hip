@@ -159,14 +159,14 @@ __global__ void __launch_bounds__(256) k_monomial(T *__restrict__ sv, MonoArgs<T
 #pragma unroll
     for (int j = 0; j < K; ++j)
       if (uint32_t(j) < m.n_active)
-        x[j] = ld_amp(amp + base + local_delta<KL>(m.col[j], g.q));
+        x[j] = amp[base + local_delta<KL>(m.col[j], g.q)];
 #pragma unroll
     for (int j = 0; j < K; ++j)
       if (uint32_t(j) < m.n_active) {
         VT o;
         o.x = m.re[j] * x[j].x - m.im[j] * x[j].y;
         o.y = m.re[j] * x[j].y + m.im[j] * x[j].x;
-        st_amp(amp + base + local_delta<KL>(m.row[j], g.q), o);
+        amp[base + local_delta<KL>(m.row[j], g.q)] = o;
       }
   }
 }

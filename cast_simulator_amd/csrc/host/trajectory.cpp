@@ -4,10 +4,14 @@
 #include "fusion.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <exception>
+#include <mutex>
 #include <numeric>
+#include <thread>
 
 namespace cast::hip {
 
@@ -258,27 +262,45 @@ TrajectoryPlan TrajectoryEngine::plan(const std::vector<TrajectorySample> &sampl
   auto &tasks = P.tasks;
   auto &pool = P.pool;
   std::map<const Segment *, uint32_t> seg_pool; // clean matrices of a segment in this plan
-  // k <= 3: interleaved complex; k = 4, 5: three real matrices Re, Im,
+  // k <= 3: interleaved complex; k >= 4: three real matrices Re, Im,
   // Re + Im (the kernel's three-product form); both in whole complex slots.
-  auto push_matrix = [&](const ComplexSquareMatrix &m) {
+  auto reserve_matrix = [&](const ComplexSquareMatrix &m) {
     const uint64_t off = pool.size() / 2;
     if (off + m.data().size() * 2 > 0xffffffffull)
       throw Error("trajectory matrix pool exceeds 2^32 complex entries");
-    if (m.edge_size() >= 16) {
-      for (const cplx &z : m.data())
-        pool.push_back(z.real());
-      for (const cplx &z : m.data())
-        pool.push_back(z.imag());
-      for (const cplx &z : m.data())
-        pool.push_back(z.real() + z.imag());
-    } else {
-      for (const cplx &z : m.data()) {
-        pool.push_back(z.real());
-        pool.push_back(z.imag());
-      }
-    }
+    pool.resize(pool.size() + (m.edge_size() >= 16 ? 3 : 2) * m.data().size());
     return uint32_t(off);
   };
+  auto write_matrix = [&pool](const ComplexSquareMatrix &m, uint32_t off) {
+    double *d = pool.data() + 2 * size_t(off);
+    const size_t sz = m.data().size();
+    if (m.edge_size() >= 16) {
+      for (size_t i = 0; i < sz; ++i) {
+        const cplx z = m.data()[i];
+        d[i] = z.real();
+        d[sz + i] = z.imag();
+        d[2 * sz + i] = z.real() + z.imag();
+      }
+    } else {
+      for (size_t i = 0; i < sz; ++i) {
+        d[2 * i] = m.data()[i].real();
+        d[2 * i + 1] = m.data()[i].imag();
+      }
+    }
+  };
+  auto push_matrix = [&](const ComplexSquareMatrix &m) {
+    const uint32_t off = reserve_matrix(m);
+    write_matrix(m, off);
+    return off;
+  };
+  // Matrices of fused gates that carry errors: their pool slots are reserved
+  // in plan order and filled afterwards on several threads.
+  struct DirtyJob {
+    uint32_t off;
+    const FusedGate *f;
+    std::vector<std::pair<uint32_t, PauliError>> errs;
+  };
+  std::vector<DirtyJob> jobs;
   auto slots_of = [](const ComplexSquareMatrix &m) {
     return uint32_t(m.edge_size() >= 16 ? 3 * m.data().size() / 2 : m.data().size());
   };
@@ -320,7 +342,12 @@ TrajectoryPlan TrajectoryEngine::plan(const std::vector<TrajectorySample> &sampl
       if (it == dirty.end()) {
         push_gate(f, off);
       } else {
-        push_gate(f, push_matrix(dirty_matrix(f, it->second)));
+        // Fill the component-product cache here: the threads only read it.
+        for (const auto &pe : it->second)
+          prefix(f, pe.first);
+        const uint32_t doff = reserve_matrix(f.m);
+        jobs.push_back({doff, &f, std::move(it->second)});
+        push_gate(f, doff);
         ++P.n_dirty_matrices;
       }
       off += slots_of(f.m);
@@ -373,6 +400,33 @@ TrajectoryPlan TrajectoryEngine::plan(const std::vector<TrajectorySample> &sampl
     if (s.errors.empty())
       clean_task[s.depth] = uint32_t(tasks.size());
     tasks.push_back(task);
+  }
+  {
+    const size_t n_threads = std::min<size_t>(
+        {size_t(16), size_t(std::max(1u, std::thread::hardware_concurrency())),
+         jobs.size() / 16 + 1});
+    std::atomic<size_t> next{0};
+    std::exception_ptr failure;
+    std::mutex failure_mu;
+    auto work = [&] {
+      try {
+        for (size_t i = next.fetch_add(1); i < jobs.size(); i = next.fetch_add(1))
+          write_matrix(dirty_matrix(*jobs[i].f, jobs[i].errs), jobs[i].off);
+      } catch (...) {
+        std::lock_guard<std::mutex> lk(failure_mu);
+        if (!failure)
+          failure = std::current_exception();
+        next.store(jobs.size());
+      }
+    };
+    std::vector<std::thread> threads;
+    for (size_t i = 1; i < n_threads; ++i)
+      threads.emplace_back(work);
+    work();
+    for (auto &t : threads)
+      t.join();
+    if (failure)
+      std::rethrow_exception(failure);
   }
   P.order.resize(tasks.size());
   std::iota(P.order.begin(), P.order.end(), 0u);
