@@ -96,6 +96,9 @@ def run_gpu(cs, n: int, precision: str, steps: int, warmup: int, max_size: int):
             low = sum(1 for q in info[kid].qubits if q < 6)
             return (f"k_tile3<{ctype}, {k}, {low}>" if f in (6, 7, 9)
                     else f"k_tile<{ctype}, {k}, {low}>")
+        if f == 1:  # dense: the store mode is a template argument
+            nts = precision == "f64" and min(info[kid].qubits) >= 3
+            return f"k_dense<{ctype}, {k}, {'true' if nts else 'false'}>"
         return f"{KERNEL[f]}<{ctype}, {k}>"
 
     per_kernel = defaultdict(list)

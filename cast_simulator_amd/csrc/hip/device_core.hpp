@@ -97,8 +97,10 @@ __device__ __forceinline__ uint64_t local_delta(uint32_t i, const uint32_t *q) {
 }
 
 // Dense update of one combo: K amplitudes read from `src`, K written to
-// `dst` (src == dst for in-place), new[r] = sum_c M[r][c] old[c].
-template <typename T, int KL>
+// `dst` (src == dst for in-place), new[r] = sum_c M[r][c] old[c].  NTS:
+// nontemporal stores at f64, which pay only when a wave's stores cover whole
+// 128-byte lines (no target below bit 3); otherwise plain stores.
+template <typename T, int KL, bool NTS = false>
 __device__ __forceinline__ void dense_combo(const typename V2<T>::type *src,
                                             typename V2<T>::type *dst, const T *__restrict__ mat,
                                             uint64_t base, const uint64_t *delta) {
@@ -126,7 +128,10 @@ __device__ __forceinline__ void dense_combo(const typename V2<T>::type *src,
     VT out;
     out.x = ar0 - ar1;
     out.y = ai0 + ai1;
-    st_amp(dst + base + delta[r], out);
+    if constexpr (NTS)
+      st_amp(dst + base + delta[r], out);
+    else
+      dst[base + delta[r]] = out;
   }
 }
 
