@@ -161,7 +161,7 @@ def test_multi_kernel_same_manager_and_dedup():
 SHAPES = [
     [0], [7], [13], [0, 1], [0, 9], [5, 12], [0, 1, 2], [1, 4, 11], [8, 9, 10],
     [0, 1, 2, 3], [0, 5, 6, 13], [9, 10, 11, 12], [0, 1, 2, 3, 4], [1, 3, 5, 7, 9],
-    [2, 8, 10, 12, 13], [0, 1, 2, 3, 4, 5], [3, 4, 7, 9, 12, 13],
+    [2, 8, 10, 12, 13], [3, 6, 8, 11, 13], [0, 1, 2, 3, 4, 5], [3, 4, 7, 9, 12, 13],
 ]
 
 
