@@ -477,6 +477,42 @@ def test_roundtrip_at_30_qubits(f32):
     assert np.all(np.abs(zz - 1.0) < tol)
 
 
+ROUNDTRIP_SHAPES = [[0, 1, 2, 3, 4], [5, 11, 17, 23, 29], [1, 9, 15, 26], [3, 6, 8, 11, 13, 27]]
+
+
+@pytest.mark.parametrize("variant,flags", VARIANTS[1:],
+                         ids=["direct", "tile", "tile-swizzled", "tile-gauss",
+                              "tile-gauss-swizzled", "tile-xcd", "tile-gauss-xcd", "mfma"])
+@pytest.mark.parametrize("f32", [False, True], ids=["f64", "f32"])
+def test_roundtrip_at_30_qubits_every_family(f32, variant, flags):
+    """Each kernel family forced through the C ABI at the headline size: the
+    gates (k = 4..6, low, high and mixed targets) and then their adjoints in
+    reverse order return |0...0>; norm and <Z_i Z_j> checked on the device,
+    so the 16 GiB state never crosses PCIe."""
+    n, p = 30, (abi.F32 if f32 else abi.F64)
+    print(f"[parity] 30q roundtrip variant={variant} flags={flags} f32={f32}", flush=True)
+    fwd = [cs.QuantumGate.random_unitary(q, 950 + i) for i, q in enumerate(ROUNDTRIP_SHAPES)]
+    mats = [np.asarray(g.matrix()) for g in fwd] + [np.asarray(g.matrix()).conj().T
+                                                    for g in fwd[::-1]]
+    qs = [list(g.qubits()) for g in fwd] + [list(g.qubits()) for g in fwd[::-1]]
+    d = abi.sv_alloc(n, p)
+    plans = []
+    try:
+        abi.sv_zero(d, n, p)
+        for m, q in zip(mats, qs):
+            plans.append(abi.plan_create(m, q, p, variant=variant, flags=flags))
+            abi.gate_apply(plans[-1], d, n)
+        abi.stream_sync(None)
+        tol = 1e-5 if f32 else 1e-12
+        assert abs(abi.norm_squared(d, n, p) - 1.0) < tol
+        zz = abi.expect_zz(d, n, p, [(0, 1), (4, 5), (28, 29), (0, 29), (3, 27)])
+        assert np.all(np.abs(zz - 1.0) < tol), zz
+    finally:
+        for pl in plans:
+            abi.plan_destroy(pl)
+        abi.sv_free(d)
+
+
 @pytest.mark.parametrize("f32", [False, True], ids=["f64", "f32"])
 @pytest.mark.parametrize("qubits", [[0, 1, 2, 3, 4, 5], [2, 9, 14, 17, 20, 23], [18, 19, 20, 21, 22, 23]],
                          ids=["low", "mixed", "high"])
