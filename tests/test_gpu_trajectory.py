@@ -126,20 +126,26 @@ def test_noiseless_limit_equals_statevector_expectations():
     assert abs(r.min_norm_sq - 1.0) < 1e-12 and abs(r.max_norm_sq - 1.0) < 1e-12
 
 
-def test_f32_matches_f64_at_24_qubits():
+@pytest.mark.parametrize("kmax32", [4, 6])
+def test_f32_matches_f64_at_24_qubits(kmax32):
+    """The benchmark's statevector size: f32 trajectories (k <= 4 on wave
+    tiles, or k <= 6 with the k = 5, 6 gates on matrix cores) against the
+    f64 trajectories of the same samples (k <= 4), state by state."""
     n, D = 24, 12
     lc = cs.LayeredCircuit.random_layers(n, D, 2026)
-    S = cs.sample_trajectories(lc, cs.NoiseSpec(1e-3, 10, D, 4), 6)
+    S = cs.sample_trajectories(lc, cs.NoiseSpec(1e-2, 10, D, 4), 6)
     pairs = [(i, i + 1) for i in range(n - 1)]
     flat = [q for p in pairs for q in p]
-    print("[traj] n=24 f32 vs f64 dump, 6 trajectories", flush=True)
+    print(f"[traj] n=24 f32 (k<={kmax32}) vs f64 dump, 6 trajectories", flush=True)
     a = cs.TrajectoryEngine(lc, F64, flat).run(S, cs.TrajectoryTuning(4, 4, 256, 2, 0), True)
-    b = cs.TrajectoryEngine(lc, F32, flat).run(S, cs.TrajectoryTuning(4, 4, 512, 1, 0), True)
+    b = cs.TrajectoryEngine(lc, F32, flat).run(S, cs.TrajectoryTuning(4, kmax32, 512, 1, 0), True)
+    assert b.n_dirty_matrices > 0
     da, db = a.dump().reshape(a.n_tasks, -1), b.dump().reshape(b.n_tasks, -1)
     for t in range(len(S)):
         x, y = to_complex(da[a.task_of[t]]), to_complex(db[b.task_of[t]])
         fid = abs(np.vdot(x, y)) ** 2 / (np.vdot(x, x).real * np.vdot(y, y).real)
         assert fid > 1 - 1e-6, f"trajectory {t}: fidelity {fid}"
+        assert rel_l2(da[a.task_of[t]], db[b.task_of[t]].astype(np.float64)) <= 1e-5
         assert abs(a.traj_zz[t] - b.traj_zz[t]) < 1e-5
     assert abs(a.min_norm_sq - 1) < 1e-10 and abs(b.max_norm_sq - 1) < 1e-5
 
