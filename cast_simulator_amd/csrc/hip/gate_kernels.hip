@@ -417,19 +417,27 @@ void launch_mono_t(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &g
 
 template <typename T, int KL>
 void launch_mfma_t(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &g) {
-  // Grid cap: 128 workgroups per CU, so each wave takes a few tens of tiles
-  // (the three real matrices are loaded into registers once per wave).
-  // Measured on the 29 k = 5 gates of the 30-qubit f64 headline circuit:
-  // 4898 GB/s aggregate at 8 per CU, 5118 at 32, 5232 at 128, 5049 at 512;
-  // long grid-stride loops stream worse than full grids.
-  static const unsigned per_cu = [] {
+  // Grid: the three real matrices are loaded into registers once per wave,
+  // and long grid-stride loops stream worse than full grids.  At f64 the
+  // grid gives each wave about 16 tiles, between 8 and 128 workgroups per
+  // CU; at f32, 8 per CU.  Measured on the 29 k = 5 gates of the 30-qubit
+  // f64 headline circuit: 4898 GB/s aggregate at 8 per CU (256 tiles per
+  // wave), 5118 at 32, 5232 at 128 (16 tiles per wave), 5049 at 512; the
+  // serving workload (12-28 qubits, f32 and f64): 79.3 circuits/s with 8
+  // per CU, 72.3 with 128.
+  static const unsigned per_cu_env = [] {
     const char *e = std::getenv("CAST_HIP_MFMA_BLOCKS_PER_CU");
-    long v = e && *e ? std::strtol(e, nullptr, 10) : 128;
-    return unsigned(v > 0 && v <= 1024 ? v : 128);
+    long v = e && *e ? std::strtol(e, nullptr, 10) : 0;
+    return unsigned(v > 0 && v <= 1024 ? v : 0);
   }();
   const uint64_t tiles = g.combos / 16;
   const uint64_t need = (tiles + 3) / 4; // 4 waves per block
-  const uint64_t cap = uint64_t(std::max(1, device_props().n_cu)) * per_cu;
+  const uint64_t n_cu = uint64_t(std::max(1, device_props().n_cu));
+  uint64_t cap = n_cu * 8;
+  if (per_cu_env)
+    cap = n_cu * per_cu_env;
+  else if (sizeof(T) == 8)
+    cap = std::min(n_cu * 128, std::max(cap, tiles / 64));
   const unsigned grid = unsigned(std::max<uint64_t>(1, std::min(need, cap)));
   hipLaunchKernelGGL((k_mfma<T, KL>), dim3(grid), dim3(kBlock), 0, s, static_cast<T *>(sv),
                      static_cast<const T *>(p.d_mat3), g, tiles);
