@@ -448,7 +448,21 @@ template <typename T>
 void launch_mfma6_t(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &g) {
   const uint64_t tiles = g.combos / 16;
   const uint64_t need = (tiles + kMfma6Block / 64 - 1) / (kMfma6Block / 64);
-  const uint64_t cap = uint64_t(std::max(1, device_props().n_cu)) * 2; // LDS: 2 blocks per CU
+  // 2 workgroups per CU are resident (LDS).  The grid gives each wave about
+  // 256 tiles, between 2 and 32 workgroups per CU (each workgroup stages
+  // the matrices in LDS once).  Measured: k = 6 at 34 qubits f32 136.3 ->
+  // 138.5 TFLOP/s and at 33 qubits f64 74.8 -> 75.4 going from 2 to 32 per
+  // CU, while the serving workload (12-28 qubits) served 79.4 circuits/s
+  // with 2 per CU and 77.4 with 32.  CAST_HIP_MFMA6_BLOCKS_PER_CU fixes the
+  // cap for measurements.
+  static const unsigned per_cu_env = [] {
+    const char *e = std::getenv("CAST_HIP_MFMA6_BLOCKS_PER_CU");
+    long v = e && *e ? std::strtol(e, nullptr, 10) : 0;
+    return unsigned(v > 0 && v <= 1024 ? v : 0);
+  }();
+  const uint64_t n_cu = uint64_t(std::max(1, device_props().n_cu));
+  const uint64_t cap = per_cu_env ? n_cu * per_cu_env
+                                  : std::min(n_cu * 32, std::max(n_cu * 2, tiles / 2048));
   const unsigned grid = unsigned(std::max<uint64_t>(1, std::min(need, cap)));
   hipLaunchKernelGGL((k_mfma6<T>), dim3(grid), dim3(kMfma6Block), 0, s, static_cast<T *>(sv),
                      static_cast<const T *>(p.d_mat3), g, tiles);
