@@ -141,6 +141,8 @@ def run_gpu(cs, n: int, precision: str, steps: int, warmup: int, max_size: int):
     return {
         "autotune": {"passes": rounds, "seconds": round(t_tune, 2),
                      "kernels_chosen": dict(sorted(fam_hist.items()))},
+        "mgr": mgr,
+        "ids": ids,
         "gates": gates,
         "n_gates": n_gates,
         "k_hist": dict(sorted(hist.items())),
@@ -157,6 +159,101 @@ def run_gpu(cs, n: int, precision: str, steps: int, warmup: int, max_size: int):
         "profile": src,
         "crossover_ai": prof.crossover_ai,
     }
+
+
+def fidelity_f32_vs_f64(cs, r64, r32, n: int):
+    """One pass of the headline circuit at f64 and at f32 from |0...0>, with
+    the kernels the timed runs used (each precision fused by its own cost
+    model), both statevectors resident (16 + 8 GiB at 30 qubits): fidelity
+    |<f32|f64>|^2 / (<f32|f32><f64|f64>) and the norms, reduced in f64 on the
+    device."""
+    s64 = cs.HipStatevector(n, cs.Precision.F64)
+    s32 = cs.HipStatevector(n, cs.Precision.F32)
+    for sv, r in ((s64, r64), (s32, r32)):
+        sv.zero()
+        for i in r["ids"]:
+            r["mgr"].apply(i, sv)
+        r["mgr"].sync(False)
+    out = {"fidelity_vs_f64": s64.fidelity(s32),
+           "norm_error_f32": abs(s32.norm_squared() - 1.0),
+           "norm_error_f64": abs(s64.norm_squared() - 1.0)}
+    del s64, s32
+    return out
+
+
+def run_qft20(cs, n: int = 20, reps: int = 20):
+    """Config 1: QFT with final swaps on 20 qubits (220 gates), size-only
+    fusion <= 2 (Phase 1 only), on a seeded complex Gaussian f64 state.  GPU:
+    the fused list on the device, kernel time from events and wall time per
+    pass, over `reps` passes after one untimed pass whose result is kept for
+    the parity check against the CPU baseline (cpu_qft20)."""
+    import numpy as np
+
+    g = cs.CircuitGraph.from_circuit(cs.QuantumCircuit.qft(n, True))
+    cs.optimize(g, cs.FusionConfig.size_only(2))
+    gates = g.gates_in_row_order()
+    init = np.asarray(cs.random_statevector(n, SEED))
+    mgr = cs.HipKernelManager(cs.HipKernelGenSpec.f64())
+    ids = [mgr.generate(x) for x in gates]
+    sv = cs.HipStatevector(n, cs.Precision.F64)
+    sv.upload(init)
+    for i in ids:
+        mgr.apply(i, sv)
+    mgr.sync(False)
+    first = np.asarray(sv.download())
+    gpu = 0.0
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for i in ids:
+            mgr.apply(i, sv)
+        kernels, _ = mgr.sync(True)
+        gpu += sum(t for _, t in kernels)
+    wall = time.perf_counter() - t0
+    b = 2 * (1 << n) * 2 * 8
+    hist = defaultdict(int)
+    for x in gates:
+        hist[x.n_qubits()] += 1
+    return {
+        "workload": f"{n}-qubit QFT with swaps (220 gates), size-only fusion <= 2, seeded complex "
+                    f"Gaussian f64 state (seed {SEED}), one resident statevector (16 MiB)",
+        "fused_gates": len(ids), "k_histogram": dict(sorted(hist.items())),
+        "gpu_kernel_ms_per_pass": round(gpu / reps * 1e3, 3),
+        "gpu_wall_ms_per_pass": round(wall / reps * 1e3, 3),
+        "gbs_per_gate_kernel": round(len(ids) * b * reps / gpu / 1e9, 1),
+        "passes": reps,
+        "_gates": gates, "_init": init, "_first": first,
+    }
+
+
+def cpu_qft20(q, threads: int, min_s: float = 2.0):
+    """CPU leg of config 1: the oracle's SIMD split-layout port (a
+    restatement of the reference CPU path) over the whole fused list on
+    `threads` threads, repeated until `min_s` seconds; its first pass is
+    compared amplitude by amplitude with the GPU's."""
+    import numpy as np
+
+    import oracle
+
+    gates, init, first = q["_gates"], q["_init"], q["_first"]
+    n = (init.size // 2).bit_length() - 1
+    sv = oracle.SimdStatevector(n, f32=False)
+    sv.load(init)
+    mats = [(np.asarray(x.matrix()), list(x.qubits())) for x in gates]
+    for m, qs in mats:
+        sv.apply(m, qs, threads)
+    ref = sv.interleaved()
+    za, zb = ref[0::2] + 1j * ref[1::2], first[0::2] + 1j * first[1::2]
+    rel = float(np.linalg.norm(za - zb) / np.linalg.norm(za))
+    passes, t0 = 0, time.perf_counter()
+    while True:
+        for m, qs in mats:
+            sv.apply(m, qs, threads)
+        passes += 1
+        if time.perf_counter() - t0 >= min_s:
+            break
+    dt = (time.perf_counter() - t0) / passes
+    return {"ms_per_pass": round(dt * 1e3, 3), "threads": threads, "passes": passes,
+            "gpu_vs_cpu_rel_l2": rel, "max_abs_diff": float(np.max(np.abs(za - zb)))}
 
 
 def run_dense_large(cs, n64: int = 33, n32: int = 34, depth: int = 12):
@@ -234,16 +331,25 @@ def run_dense_large(cs, n64: int = 33, n32: int = 34, depth: int = 12):
                 "peak": peak, "unit": "TFLOP/s", "frac": round(top["tflops"] / peak, 4),
                 "gbs": top["gbs"], "traffic": None}
     return {
-        "metric": "config 3: per-gate GB/s and algorithmic TFLOP/s, 33q f64 / 34q f32, k<=6",
-        "value": r64["gbs_per_gate"], "unit": "GB/s", "n_gpus": 1, "steps": 1, "warmup": 0,
-        "ms_per_step": round(r64["wall_s"] * 1e3, 1), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"{n64}-qubit f64 and {n32}-qubit f32 random-layer circuit, depth "
-                               f"{depth}, seed {SEED}, size-only fusion k<=6, automatic kernel "
-                               "family"},
+        "workload": f"{n64}-qubit f64 and {n32}-qubit f32 random-layer circuit, depth {depth}, "
+                    f"seed {SEED}, size-only fusion k<=6, automatic kernel family; fidelity of "
+                    f"the {n64}-qubit f32 state against the f64 one (both resident)",
         "f64": r64, "f32": r32, "f32_same_circuit_as_f64": r32s,
         "fidelity_f32_vs_f64": fid,
         "roofline": roof(r64, "f64"), "roofline_f32": roof(r32, "f32"),
+        "flop_convention": "tflops = 8 K 2^n true real flops per dense k-qubit gate; the "
+                           "reference's GFLOP/s unit (src/profile.rs:436) is half of it",
+    }
+
+
+def dense_large_line(d):
+    """The --workload dense-large JSON line around run_dense_large's result."""
+    return {
+        "metric": "config 3: per-gate GB/s and algorithmic TFLOP/s, 33q f64 / 34q f32, k<=6",
+        "value": d["f64"]["gbs_per_gate"], "unit": "GB/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": round(d["f64"]["wall_s"] * 1e3, 1), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": d["workload"]}, **{k: v for k, v in d.items() if k != "workload"},
     }
 
 
@@ -358,26 +464,36 @@ def run_trajectories(cs, n: int, count: int, tune_sample: int, plain_sample: int
 
     # Plain per-circuit f64 GPU path on the first trajectories: explicit gate
     # list (Paulis inserted), fused to k <= 5 by the hardware-adaptive model,
-    # one statevector each.
+    # one statevector each; its kernel time from events and its end-to-end
+    # time (fusion + plans + kernels).  The batched f32 states of the same
+    # trajectories are dumped 8 at a time and compared with the f64 ones.
     prof, _ = hw_profile(cs, "f64")
     sim = cs.Simulator.f64()
     sim.manager().set_autotune(False)
-    t_plain = 0.0
+    t_plain = gpu_plain = 0.0
+    plain_passes = 0
     fids = []
     sub = S[:plain_sample]
-    rd = eng.run(sub, best, True)
-    dump = rd.dump().reshape(rd.n_tasks, -1)
-    for t, s in enumerate(sub):
-        tp = time.perf_counter()
-        g = cs.CircuitGraph.from_circuit(cs.trajectory_circuit(lc, s))
-        cs.optimize(g, cs.FusionConfig.hardware_adaptive(prof, 5, 1e-12))
-        sv = sim.simulate(g)
-        t_plain += time.perf_counter() - tp
-        x = np.asarray(sv.download())
-        y = dump[rd.task_of[t]]
-        zx = x[0::2] + 1j * x[1::2]
-        zy = y[0::2] + 1j * y[1::2]
-        fids.append(abs(np.vdot(zx, zy)) ** 2 / (np.vdot(zx, zx).real * np.vdot(zy, zy).real))
+    for b0 in range(0, len(sub), 8):
+        part = sub[b0:b0 + 8]
+        rd = eng.run(part, best, True)
+        dump = rd.dump().reshape(rd.n_tasks, -1)
+        for t, s in enumerate(part):
+            tp = time.perf_counter()
+            g = cs.CircuitGraph.from_circuit(cs.trajectory_circuit(lc, s))
+            cs.optimize(g, cs.FusionConfig.hardware_adaptive(prof, 5, 1e-12))
+            sv = cs.HipStatevector(n, cs.Precision.F64)
+            sv.zero()
+            kernels, _ = sim.run(g, sv)
+            t_plain += time.perf_counter() - tp
+            gpu_plain += sum(x for _, x in kernels)
+            plain_passes += len(kernels)
+            x = np.asarray(sv.download())
+            y = dump[rd.task_of[t]]
+            zx = x[0::2] + 1j * x[1::2]
+            zy = y[0::2] + 1j * y[1::2]
+            fids.append(abs(np.vdot(zx, zy)) ** 2 / (np.vdot(zx, zx).real * np.vdot(zy, zy).real))
+        del dump, rd
     plain_cps = len(sub) / t_plain
     return {
         "workload": f"{n}-qubit random-layer circuit, 40 layers, seed {SEED}; {count} trajectories, "
@@ -406,7 +522,23 @@ def run_trajectories(cs, n: int, count: int, tune_sample: int, plain_sample: int
         "plain_f64_circuits_per_s": round(plain_cps, 2),
         "plain_sample": len(sub),
         "speedup_vs_plain_f64": round(count / r.total_s / plain_cps, 2),
+        "speedup_split": {
+            "kernel_s_per_trajectory_plain_f64": gpu_plain / len(sub),
+            "kernel_s_per_trajectory_batched_f32": r.gpu_s / count,
+            "device_speedup": round(gpu_plain / len(sub) / (r.gpu_s / count), 2),
+            "gate_passes_per_trajectory_plain": round(plain_passes / len(sub), 2),
+            "gate_passes_per_trajectory_batched": round(r.n_gate_passes / count, 2),
+            "tasks_per_trajectory": round(r.n_tasks / count, 4),
+            "host_s_per_trajectory_plain": (t_plain - gpu_plain) / len(sub),
+            "host_s_per_trajectory_batched": (r.plan_s + r.checkpoint_s) / count,
+            "note": "plain = per-circuit f64 path on the first sample trajectories (their own "
+                    "gate lists fused to k<=5); batched = the whole launch divided by the "
+                    "trajectory count; device_speedup = kernel time per trajectory, plain over "
+                    "batched: it includes the shared clean prefixes (gate passes per "
+                    "trajectory) and f32 vs f64 bytes",
+        },
         "fidelity_vs_f64_min": float(min(fids)),
+        "fidelity_vs_f64_median": float(np.median(fids)),
         "fidelity_sample": len(fids),
     }
 
@@ -421,35 +553,45 @@ def run_serving(cs, count: int, rate: float, capacity_sample: int):
     `capacity_sample` requests all queued at t = 0, served with overlap,
     against the plain per-circuit path: the same circuits in f64, one at a
     time on one stream."""
-    reqs = cs.poisson_requests(count, rate, 12, 28, 10, 50, 0.5, SEED)
+    import numpy as np
 
-    def serve(rs, streams, threads, realtime):
+    reqs = cs.poisson_requests(count, rate, 12, 28, 10, 50, 0.5, SEED)
+    streams, threads = 4, 16
+
+    def serve(rs, n_streams, realtime, shadow=False):
         cfg = cs.ServeConfig()
-        cfg.streams, cfg.compile_threads, cfg.realtime = streams, threads, realtime
+        cfg.streams, cfg.compile_threads, cfg.realtime = n_streams, threads, realtime
+        cfg.f64_shadow = shadow
         return cs.Server(cfg).run(rs)
 
-    res, st = serve(reqs, 8, 16, True)
+    res, st = serve(reqs, streams, True)
     log(f"  serving: {count} requests at {rate}/s: {st['throughput']:.1f}/s, "
         f"p50 {st['p50_s'] * 1e3:.1f} ms, p99 {st['p99_s'] * 1e3:.1f} ms")
     sub = reqs[:capacity_sample]
-    _, cap = serve(sub, 8, 16, False)
-    plain = []
-    for r in sub:
-        plain.append(cs.ServeRequest(r.id, r.n_qubits, r.depth, r.seed, cs.Precision.F64, 0.0))
-    _, pl = serve(plain, 1, 1, False)
+    _, cap = serve(sub, streams, False)
+    # Plain per-circuit path: the same circuits in f64, one at a time on one
+    # stream, compiled ahead by the same host threads.
+    _, pl = serve([r.with_precision(cs.Precision.F64) for r in sub], 1, False)
     log(f"  capacity: {cap['throughput']:.1f} circuits/s mixed, plain f64 serial "
         f"{pl['throughput']:.1f} circuits/s")
     # Latency below saturation: a second stream at 70 % of the measured
     # capacity (the offered load at `rate` exceeds what the HBM can serve).
     sus_rate = round(0.7 * cap["throughput"], 1)
     sus_reqs = cs.poisson_requests(min(300, count), sus_rate, 12, 28, 10, 50, 0.5, SEED + 1)
-    _, sus = serve(sus_reqs, 8, 16, True)
+    _, sus = serve(sus_reqs, streams, True)
     log(f"  at {sus_rate}/s: p50 {sus['p50_s'] * 1e3:.1f} ms, p99 {sus['p99_s'] * 1e3:.1f} ms")
+    # Accuracy pass: every request again, each f32 one with its f64 replay on
+    # the device (checking mode, not timed).
+    shadow, _ = serve(reqs, streams, False, shadow=True)
+    fid = [o.fidelity_vs_f64 for o in shadow if o.precision == cs.Precision.F32]
+    same = all(np.array_equal(np.asarray(a.zz), np.asarray(b.zz)) and a.norm_sq == b.norm_sq
+               for a, b in zip(sorted(res, key=lambda o: o.id), sorted(shadow, key=lambda o: o.id)))
+    log(f"  f32 requests: fidelity vs f64 min {min(fid):.12f} over {len(fid)}")
     norms = [abs(o.norm_sq - 1.0) for o in res]
     return {
         "workload": f"{count} requests, Poisson arrivals at {rate}/s (seed {SEED}), "
                     "random-layer circuits n ~ U{12..28}, depth ~ U{10..50}, f32/f64 50/50, "
-                    "fused to k<=6, resident statevectors, admission by free HBM",
+                    "fused to k<=6, resident statevectors, admission by free HBM, 4 streams",
         "circuits_per_s": round(st["throughput"], 2),
         "p50_ms": round(st["p50_s"] * 1e3, 2),
         "p99_ms": round(st["p99_s"] * 1e3, 2),
@@ -463,6 +605,12 @@ def run_serving(cs, count: int, rate: float, capacity_sample: int):
         "plain_f64_serial_circuits_per_s": round(pl["throughput"], 2),
         "speedup_vs_plain_f64": round(cap["throughput"] / pl["throughput"], 2),
         "max_norm_error": max(norms),
+        "streams": streams,
+        "compile_threads": threads,
+        "fidelity_vs_f64_min": float(min(fid)),
+        "fidelity_vs_f64_median": float(np.median(fid)),
+        "fidelity_sample": len(fid),
+        "results_identical_in_accuracy_pass": bool(same),
         "below_saturation": {"rate": sus_rate, "requests": len(sus_reqs),
                              "circuits_per_s": round(sus["throughput"], 2),
                              "p50_ms": round(sus["p50_s"] * 1e3, 2),
@@ -489,14 +637,40 @@ def run_trajectory_launch(cs, n: int, count: int, tuning):
             "kernel_gbs": alg / r.gpu_s / 1e9}
 
 
-def cpu_baseline(gates, n_gpu: int, budget_s: float = 20.0):
+def cpu_info() -> dict:
+    """CPU model and the cores this process may run on: logical CPUs in its
+    affinity set, and distinct (package, core) pairs among them."""
+    try:
+        allowed = set(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = set(range(os.cpu_count() or 1))
+    model, phys = platform.machine(), set()
+    try:
+        for block in open("/proc/cpuinfo").read().split("\n\n"):
+            f = dict(line.split(":", 1) for line in block.splitlines() if ":" in line)
+            f = {k.strip(): v.strip() for k, v in f.items()}
+            if "processor" not in f:
+                continue
+            model = f.get("model name", model)
+            if int(f["processor"]) in allowed:
+                phys.add((f.get("physical id", "0"), f.get("core id", f["processor"])))
+    except OSError:
+        pass
+    return {"model": model, "logical": len(allowed), "physical": len(phys) or len(allowed)}
+
+
+def cpu_baseline(gates, n_gpu: int, budget_s: float = 12.0):
     """The oracle's SIMD split-layout kernels (a restatement of the
-    reference CPU path) on this host's logical CPUs, over the first fused
-    gates of the same circuit, on the same qubit count when host memory
-    allows (else 28 qubits)."""
+    reference CPU path) over the first fused gates of the same circuit, on
+    the same qubit count when host memory allows (else 28 qubits): once on
+    every logical CPU of this process (the reference's default thread count,
+    src/cpu/mod.rs:22-28) and once on one thread per physical core (the
+    reference docs' advice, docs/tools.md:176-178)."""
     import numpy as np
 
     import oracle
+
+    info = cpu_info()
 
     avail = 0
     try:
@@ -506,29 +680,40 @@ def cpu_baseline(gates, n_gpu: int, budget_s: float = 20.0):
     except OSError:
         pass
     n = n_gpu if avail > 3 * (2 << n_gpu) * 8 else 28
-    threads = oracle.num_cpus()
     sv = oracle.SimdStatevector(n, f32=False)
     sv.buf[0] = 1.0
-    warm = gates[0]
-    sv.apply(np.asarray(warm.matrix()), list(warm.qubits()), threads)  # page-in, untimed
-    done, t0 = 0, time.perf_counter()
-    for g in gates[1:]:
-        if max(g.qubits()) >= n:
-            continue
-        sv.apply(np.asarray(g.matrix()), list(g.qubits()), threads)
-        done += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    dt = time.perf_counter() - t0
     b = 2 * (1 << n) * 2 * 8
+
+    def timed(threads):
+        warm = gates[0]
+        sv.apply(np.asarray(warm.matrix()), list(warm.qubits()), threads)  # untimed
+        done, t0 = 0, time.perf_counter()
+        for g in gates[1:]:
+            if max(g.qubits()) >= n:
+                continue
+            sv.apply(np.asarray(g.matrix()), list(g.qubits()), threads)
+            done += 1
+            if time.perf_counter() - t0 > budget_s:
+                break
+        return done, time.perf_counter() - t0
+
+    logical = oracle.num_cpus()
+    d_l, t_l = timed(logical)
+    physical = max(1, min(info["physical"], logical))
+    d_p, t_p = timed(physical) if physical != logical else (d_l, t_l)
     return {
-        "value": done * b / dt / 1e9,
+        "value": d_l * b / t_l / 1e9,
         "unit": "GB/s",
-        "cores": threads,
+        "cores": logical,
         "kind": "port",
-        "sample": f"{done} fused gates (k<=5) of the same circuit on a {n}-qubit f64 split-layout "
-                  f"statevector, {threads} threads, {dt:.1f} s",
-        "cpu": platform.processor() or platform.machine(),
+        "sample": f"{d_l} fused gates (k<=5) of the same circuit on a {n}-qubit f64 split-layout "
+                  f"statevector, {logical} threads (every logical CPU of this process), "
+                  f"{t_l:.1f} s",
+        "cpu": info["model"],
+        "logical_cpus": info["logical"],
+        "physical_cores": info["physical"],
+        "physical_core_run": {"threads": physical, "gbs": round(d_p * b / t_p / 1e9, 2),
+                              "gates": d_p, "seconds": round(t_p, 1)},
     }
 
 
@@ -550,6 +735,9 @@ def main() -> int:
                     help="chunk_layers,fusion_max,block,slots_per_cu,waves_per_simd for "
                          "--workload trajectories")
     ap.add_argument("--no-traj", action="store_true")
+    ap.add_argument("--no-large", action="store_true",
+                    help="skip the 33q f64 / 34q f32 dense-fusion object (needs 192 GiB)")
+    ap.add_argument("--no-qft", action="store_true", help="skip the 20-qubit QFT object")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the marginal-probability and density-matrix measurements")
@@ -581,7 +769,7 @@ def main() -> int:
         log(f"[bench] copy probe: {nbytes} bytes device-to-device")
 
     if args.workload == "dense-large":
-        print(json.dumps(run_dense_large(cs)), flush=True)
+        print(json.dumps(dense_large_line(run_dense_large(cs))), flush=True)
         return 0
     if args.workload == "trajectories":
         b, k, blk, spc, occ = (int(x) for x in args.traj_tuning.split(","))
@@ -594,15 +782,25 @@ def main() -> int:
     r64 = run_gpu(cs, args.qubits, "f64", args.steps, args.warmup, args.max_size)
     log(f"[bench] f64: {r64['gbs_per_gate']:.1f} GB/s per gate, {r64['n_gates']} gates, "
         f"{r64['ms_per_step']:.1f} ms/step, per-k {r64['per_k_gbs']}")
-    r32 = None
+    r32 = fid = None
     if not args.no_f32:
         r32 = run_gpu(cs, args.qubits, "f32", args.steps, args.warmup, args.max_size)
         log(f"[bench] f32: {r32['gbs_per_gate']:.1f} GB/s per gate, per-k {r32['per_k_gbs']}")
+        fid = fidelity_f32_vs_f64(cs, r64, r32, args.qubits)
+        log(f"[bench] f32 vs f64 fidelity {fid['fidelity_vs_f64']:.15f}")
+    qft = None
+    if not args.no_qft:
+        log("[bench] 20-qubit QFT (fusion <= 2)")
+        qft = run_qft20(cs)
+    large = None
+    if not args.no_large:
+        log("[bench] dense fusion at 33q f64 / 34q f32")
+        large = run_dense_large(cs)
     traj = None
     if not args.no_traj:
         log(f"[bench] trajectories: {args.trajectories} x {args.traj_qubits}q")
         traj = run_trajectories(cs, args.traj_qubits, args.trajectories,
-                                min(1024, args.trajectories), 8)
+                                min(1024, args.trajectories), min(64, args.trajectories))
     serve = None
     if not args.no_serve:
         log(f"[bench] serving: {args.requests} requests at {args.rate}/s")
@@ -618,6 +816,12 @@ def main() -> int:
         log("[bench] cpu baseline ...")
         cpu = cpu_baseline(r64["gates"], args.qubits)
         log(f"[bench] cpu: {cpu['value']:.2f} GB/s ({cpu['sample']})")
+        if qft:
+            qft["cpu"] = cpu_qft20(qft, cpu["cores"])
+            qft["cpu"]["kind"] = "port"
+            qft["cpu_physical_cores"] = cpu_qft20(qft, cpu["physical_core_run"]["threads"])
+            log(f"[bench] qft20 cpu: {qft['cpu']['ms_per_pass']} ms/pass, gpu "
+                f"{qft['gpu_wall_ms_per_pass']} ms/pass, rel L2 {qft['cpu']['gpu_vs_cpu_rel_l2']:.2e}")
 
     dom = r64["dominant"]
     # HBM bytes per launch of the dominant kernel, from the committed PMC
@@ -669,13 +873,21 @@ def main() -> int:
         "amplitude_updates_per_s": round(r64["gate_updates_per_s"] * (1 << args.qubits), 0),
         "cpu_baseline": cpu,
     }
+    if fid:
+        line["fidelity_f32_vs_f64"] = fid["fidelity_vs_f64"]
     if r32:
         line["f32"] = {
             "gbs_per_gate": round(r32["gbs_per_gate"], 2),
             "ms_per_step": round(r32["ms_per_step"], 3),
             "per_kernel_gbs": {k: round(v, 1) for k, v in r32["per_k_gbs"].items()},
             "fused_gates": r32["n_gates"],
+            "k_histogram": r32["k_hist"],
+            **(fid or {}),
         }
+    if qft:
+        line["qft20"] = {k: v for k, v in qft.items() if not k.startswith("_")}
+    if large:
+        line["dense_large"] = large
     if traj:
         line["trajectories"] = traj
     if serve:

@@ -166,6 +166,9 @@ _SIGS = {
     "cast_hip_traj_run": (_i, [C.POINTER(TrajLaunch), _vp, _err, _sz]),
     "cast_hip_inner_product": (
         _i, [_u64, _u8, _u64, _u8, _u32, C.POINTER(C.c_double), _vp, _err, _sz]),
+    "cast_hip_inner_product_scratch_bytes": (_sz, []),
+    "cast_hip_inner_product_async": (
+        _i, [_u64, _u8, _u64, _u8, _u32, _u64, _u64, _vp, _err, _sz]),
     "cast_hip_marginal_probabilities": (
         _i, [_u64, _u32, _u8, C.POINTER(_u32), _sz, C.POINTER(C.c_double), _vp, _err, _sz]),
     "cast_hip_dm_populations": (
@@ -395,6 +398,15 @@ def inner_product(a: int, prec_a: int, b: int, prec_b: int, n_qubits: int,
     out = (C.c_double * 2)()
     _call("cast_hip_inner_product", a, prec_a, b, prec_b, n_qubits, out, stream)
     return complex(out[0], out[1])
+
+
+def inner_product_scratch_bytes() -> int:
+    return int(lib().cast_hip_inner_product_scratch_bytes())
+
+
+def inner_product_async(a: int, prec_a: int, b: int, prec_b: int, n_qubits: int, out: int,
+                        scratch: int, stream: int | None = None) -> None:
+    _call("cast_hip_inner_product_async", a, prec_a, b, prec_b, n_qubits, out, scratch, stream)
 
 
 def marginal_probabilities(sv: int, n_qubits: int, precision: int, measured,

@@ -741,6 +741,26 @@ int cast_hip_inner_product(uint64_t a, uint8_t precision_a, uint64_t b, uint8_t 
   });
 }
 
+size_t cast_hip_inner_product_scratch_bytes(void) { return inner_scratch_bytes(); }
+
+int cast_hip_inner_product_async(uint64_t a, uint8_t precision_a, uint64_t b, uint8_t precision_b,
+                                 uint32_t sv_n_qubits, uint64_t out, uint64_t scratch,
+                                 void *stream, char *err, size_t len) {
+  return guard(err, len, [&] {
+    check_precision(precision_a);
+    check_precision(precision_b);
+    if (!a || !b || !out || !scratch)
+      throw Error("null statevector, output or scratch pointer");
+    check_aligned(a);
+    check_aligned(b);
+    if (sv_n_qubits < 1 || sv_n_qubits > 40)
+      throw Error("statevector qubit count out of range");
+    launch_inner(reinterpret_cast<const void *>(a), precision_a, reinterpret_cast<const void *>(b),
+                 precision_b, sv_n_qubits, reinterpret_cast<double *>(out),
+                 reinterpret_cast<double *>(scratch), static_cast<hipStream_t>(stream));
+  });
+}
+
 int cast_hip_marginal_probabilities(uint64_t sv_dptr, uint32_t sv_n_qubits, uint8_t precision,
                                     const uint32_t *measured_qubits, size_t n_measured,
                                     double *out, void *stream, char *err, size_t len) {

@@ -118,6 +118,11 @@ void expect_zz(const void *sv, uint32_t n_qubits, uint8_t precision, const uint3
 // <a|b> in f64 over two statevectors of n qubits, any precisions; synchronous.
 void inner_product(const void *a, uint8_t prec_a, const void *b, uint8_t prec_b,
                    uint32_t n_qubits, double *out, hipStream_t stream);
+// The same queued on `stream` into device out[0..1] = {Re, Im}, with device
+// scratch of inner_scratch_bytes().
+size_t inner_scratch_bytes();
+void launch_inner(const void *a, uint8_t prec_a, const void *b, uint8_t prec_b, uint32_t n_qubits,
+                  double *out, double *scratch, hipStream_t stream);
 
 // Marginal probabilities over `m` measured qubits (bin bit i = address bit
 // qubits[i]) and the density-matrix diagonal, f64, into host memory;

@@ -19,7 +19,7 @@
 //   k_monomial   <= 1 nonzero per row: touches only rows that differ from the
 //                identity row, so e.g. CZ / CP move 1/4 of the bytes.
 //   k_dense_batch  k_dense over many (matrix, statevector) items, item =
-//                blockIdx.y, for batched trajectories and serving.
+//                blockIdx.z (cast_hip_gate_apply_batch).
 
 #include "cast_hip.h"
 #include "common.hpp"
@@ -326,7 +326,7 @@ __global__ void __launch_bounds__(kMfma6Block) k_mfma6(T *__restrict__ sv, const
   }
 }
 
-// ── Batched dense: item = blockIdx.y ────────────────────────────────────────
+// ── Batched dense: item = blockIdx.z ────────────────────────────────────────
 
 constexpr int kMaxBatch = 64;
 struct BatchArgs {
@@ -337,7 +337,7 @@ template <typename T, int KL>
 __global__ void __launch_bounds__(256) k_dense_batch(BatchArgs a) {
   constexpr int K = 1 << KL;
   using VT = typename V2<T>::type;
-  const BatchItem &it = a.it[blockIdx.y];
+  const BatchItem &it = a.it[blockIdx.z];
   VT *__restrict__ amp = reinterpret_cast<VT *>(it.sv);
   const T *__restrict__ mat = reinterpret_cast<const T *>(it.mat);
   uint32_t q[6];
@@ -348,7 +348,7 @@ __global__ void __launch_bounds__(256) k_dense_batch(BatchArgs a) {
 #pragma unroll
   for (int i = 0; i < K; ++i)
     delta[i] = local_delta<KL>(uint32_t(i), q);
-  const uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t c = (uint64_t(blockIdx.y) * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x;
   if (c < it.combos)
     dev::dense_combo<T, KL>(amp, amp, mat, combo_base<KL>(c, q), delta);
 }
@@ -650,10 +650,9 @@ void launch_by_k(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &g) 
 template <typename T>
 void launch_batch_by_k(const BatchArgs &a, unsigned n, uint32_t k, uint64_t max_combos,
                        hipStream_t s) {
-  const uint64_t blocks = (max_combos + kBlock - 1) / kBlock;
-  if (blocks > 0x7fffffffULL)
-    throw Error("batched statevector too large");
-  const dim3 grid(unsigned(blocks), n);
+  // Combo blocks over x and y as for a single gate (full_grid), item in z.
+  dim3 grid = full_grid(max_combos);
+  grid.z = n;
 #define CAST_HIP_BCASE(KL)                                                                       \
   case KL:                                                                                       \
     hipLaunchKernelGGL((k_dense_batch<T, KL>), grid, dim3(kBlock), 0, s, a);                    \

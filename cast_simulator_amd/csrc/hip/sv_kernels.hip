@@ -464,18 +464,17 @@ void launch_observe(const void *sv, uint32_t n_qubits, uint8_t precision, const 
   check(hipGetLastError(), "hipLaunchKernelGGL(k_sum_partials)");
 }
 
-void inner_product(const void *a, uint8_t prec_a, const void *b, uint8_t prec_b,
-                   uint32_t n_qubits, double *out, hipStream_t stream) {
-  RedScratch &s = scratch();
-  std::lock_guard<std::mutex> lock(s.mu);
-  s.ensure();
+size_t inner_scratch_bytes() { return sizeof(double) * kMaxRedGrid * 2; }
+
+void launch_inner(const void *a, uint8_t prec_a, const void *b, uint8_t prec_b, uint32_t n_qubits,
+                  double *out, double *scratch, hipStream_t stream) {
   const uint64_t n_amp = uint64_t(1) << n_qubits;
   const unsigned grid = red_grid(n_amp);
   auto go = [&](auto *pa, auto *pb) {
     using A = std::remove_const_t<std::remove_pointer_t<decltype(pa)>>;
     using B = std::remove_const_t<std::remove_pointer_t<decltype(pb)>>;
     hipLaunchKernelGGL((k_inner<A, B>), dim3(grid), dim3(kRedBlock), 0, stream, pa, pb, n_amp,
-                       s.d_partial);
+                       scratch);
   };
   if (prec_a == 0 && prec_b == 0)
     go(static_cast<const float *>(a), static_cast<const float *>(b));
@@ -486,9 +485,16 @@ void inner_product(const void *a, uint8_t prec_a, const void *b, uint8_t prec_b,
   else
     go(static_cast<const double *>(a), static_cast<const double *>(b));
   check(hipGetLastError(), "hipLaunchKernelGGL(k_inner)");
-  hipLaunchKernelGGL(k_sum_partials, dim3(2), dim3(kRedBlock), 0, stream, s.d_partial, grid, 2u,
-                     s.d_out);
+  hipLaunchKernelGGL(k_sum_partials, dim3(2), dim3(kRedBlock), 0, stream, scratch, grid, 2u, out);
   check(hipGetLastError(), "hipLaunchKernelGGL(k_sum_partials)");
+}
+
+void inner_product(const void *a, uint8_t prec_a, const void *b, uint8_t prec_b,
+                   uint32_t n_qubits, double *out, hipStream_t stream) {
+  RedScratch &s = scratch();
+  std::lock_guard<std::mutex> lock(s.mu);
+  s.ensure();
+  launch_inner(a, prec_a, b, prec_b, n_qubits, s.d_out, s.d_partial, stream);
   check(hipMemcpyAsync(s.h_out, s.d_out, 2 * sizeof(double), hipMemcpyDeviceToHost, stream),
         "hipMemcpyAsync(inner)");
   check(hipStreamSynchronize(stream), "hipStreamSynchronize(inner)");

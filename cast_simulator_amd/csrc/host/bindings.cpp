@@ -692,6 +692,28 @@ PYBIND11_MODULE(_host, m) {
            }),
            py::arg("id"), py::arg("n_qubits"), py::arg("depth"), py::arg("seed"),
            py::arg("precision"), py::arg("arrival_s") = 0.0)
+      .def_static(
+          "from_circuit",
+          [](uint32_t id, const QuantumCircuit &c, cast_precision_t prec, double arrival_s) {
+            return circuit_request(id, std::make_shared<const QuantumCircuit>(c), prec, arrival_s);
+          },
+          py::arg("id"), py::arg("circuit"), py::arg("precision"), py::arg("arrival_s") = 0.0)
+      .def_static(
+          "from_qasm",
+          [](uint32_t id, const std::string &text, cast_precision_t prec, double arrival_s) {
+            return circuit_request(
+                id, std::make_shared<const QuantumCircuit>(circuit_from_qasm(parse_qasm(text))),
+                prec, arrival_s);
+          },
+          py::arg("id"), py::arg("qasm"), py::arg("precision"), py::arg("arrival_s") = 0.0)
+      .def_property_readonly("has_circuit",
+                             [](const ServeRequest &r) { return bool(r.circuit); })
+      .def("with_precision",
+           [](const ServeRequest &r, cast_precision_t prec) {
+             ServeRequest o = r;
+             o.precision = prec;
+             return o;
+           })
       .def_readwrite("id", &ServeRequest::id)
       .def_readwrite("n_qubits", &ServeRequest::n_qubits)
       .def_readwrite("depth", &ServeRequest::depth)
@@ -710,7 +732,8 @@ PYBIND11_MODULE(_host, m) {
       .def_readwrite("budget_bytes", &ServeConfig::budget_bytes)
       .def_readwrite("reserve_bytes", &ServeConfig::reserve_bytes)
       .def_readwrite("realtime", &ServeConfig::realtime)
-      .def_readwrite("keep_states", &ServeConfig::keep_states);
+      .def_readwrite("keep_states", &ServeConfig::keep_states)
+      .def_readwrite("f64_shadow", &ServeConfig::f64_shadow);
 
   py::class_<ServeResult>(m, "ServeResult")
       .def_readonly("id", &ServeResult::id)
@@ -724,6 +747,7 @@ PYBIND11_MODULE(_host, m) {
       .def_readonly("done_s", &ServeResult::done_s)
       .def_readonly("norm_sq", &ServeResult::norm_sq)
       .def_readonly("zz", &ServeResult::zz)
+      .def_readonly("fidelity_vs_f64", &ServeResult::fidelity_vs_f64)
       .def("state", [](const ServeResult &r) {
         return py::array_t<double>(py::ssize_t(r.state.size()), r.state.data());
       });

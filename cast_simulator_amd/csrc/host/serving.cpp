@@ -18,7 +18,17 @@ namespace cast::hip {
 namespace {
 
 constexpr size_t kErr = 1024;
+// One observe call reduces at most kPairs pairs; a request of n qubits has
+// n - 1 neighbour pairs, reduced in chunks of kPairs (two chunks up to 40
+// qubits).  Device result layout per slot (doubles): chunk c at
+// c * (1 + kPairs) (norm, then its pairs), then the shadow's inner product
+// (re, im) and the shadow's norm.
 constexpr uint32_t kPairs = 32;
+constexpr uint32_t kChunks = 2;
+constexpr uint32_t kOutInner = kChunks * (1 + kPairs);
+constexpr uint32_t kOutShadowNorm = kOutInner + 2;
+constexpr uint32_t kOut = kOutShadowNorm + 1;
+constexpr uint32_t kMaxQubits = 40;
 
 double now_s() {
   using namespace std::chrono;
@@ -36,6 +46,17 @@ size_t sv_bytes_of(uint32_t n, cast_precision_t p) {
   return (size_t(2) << n) * (p == CAST_PRECISION_F32 ? 4 : 8);
 }
 
+size_t round_up_256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// Device bytes a request holds while admitted: its statevector, plus the F64
+// shadow right after it (256-byte aligned) when it has one.
+size_t request_bytes(const ServeRequest &r, bool shadow) {
+  const size_t b = sv_bytes_of(r.n_qubits, r.precision);
+  if (!shadow)
+    return b;
+  return round_up_256(b) + sv_bytes_of(r.n_qubits, CAST_PRECISION_F64);
+}
+
 double percentile(std::vector<double> v, double q) {
   if (v.empty())
     return 0.0;
@@ -45,6 +66,20 @@ double percentile(std::vector<double> v, double q) {
 }
 
 } // namespace
+
+ServeRequest circuit_request(uint32_t id, std::shared_ptr<const QuantumCircuit> circuit,
+                             cast_precision_t precision, double arrival_s) {
+  if (!circuit)
+    throw Error("circuit_request: null circuit");
+  ServeRequest r;
+  r.id = id;
+  r.n_qubits = circuit->n_qubits();
+  r.depth = uint32_t(circuit->len());
+  r.precision = precision;
+  r.arrival_s = arrival_s;
+  r.circuit = std::move(circuit);
+  return r;
+}
 
 std::vector<ServeRequest> poisson_requests(uint32_t count, double rate, uint32_t n_min,
                                            uint32_t n_max, uint32_t depth_min,
@@ -80,9 +115,9 @@ struct Server::Slot {
   // current request
   bool busy = false;
   size_t req = 0;
-  uint64_t sv = 0;
+  uint64_t sv = 0, shadow = 0; // shadow: F64 replay of an F32 request, or 0
   size_t sv_bytes = 0;
-  std::vector<void *> plans;
+  std::vector<void *> plans, shadow_plans;
 };
 
 // Statevector blocks recycled by exact size under a byte budget.
@@ -160,10 +195,11 @@ Server::Server(ServeConfig cfg) : cfg_(cfg), pool_(new Pool) {
     s->event = cast_hip_event_create(err, kErr);
     if (!s->event)
       throw Error(err);
-    s->scratch = cast_hip_malloc(scratch, err, kErr);
-    s->out = s->scratch ? cast_hip_malloc((1 + kPairs) * sizeof(double), err, kErr) : 0;
+    s->scratch = cast_hip_malloc(std::max(scratch, cast_hip_inner_product_scratch_bytes()), err,
+                                 kErr);
+    s->out = s->scratch ? cast_hip_malloc(kOut * sizeof(double), err, kErr) : 0;
     s->h_out = static_cast<double *>(
-        s->out ? cast_hip_host_alloc((1 + kPairs) * sizeof(double), err, kErr) : nullptr);
+        s->out ? cast_hip_host_alloc(kOut * sizeof(double), err, kErr) : nullptr);
     if (!s->h_out)
       throw Error(err);
   }
@@ -175,6 +211,8 @@ Server::~Server() {
     if (s->stream)
       (void)cast_hip_stream_sync(s->stream, err, kErr);
     for (void *p : s->plans)
+      cast_hip_gate_plan_destroy(p);
+    for (void *p : s->shadow_plans)
       cast_hip_gate_plan_destroy(p);
     if (s->busy && s->sv)
       cast_hip_free(s->sv);
@@ -216,16 +254,26 @@ std::vector<ServeResult> Server::run(const std::vector<ServeRequest> &reqs, Serv
                                   : avail + pool.in_use;
   pool.peak = pool.in_use;
   for (const ServeRequest &r : reqs) {
-    if (r.n_qubits < 2 || r.n_qubits > 34 || r.depth < 1)
-      throw Error("serving: request " + std::to_string(r.id) + " has an invalid shape");
-    if (sv_bytes_of(r.n_qubits, r.precision) > pool.budget)
-      throw Error("serving: request " + std::to_string(r.id) + " needs more memory than the "
-                  "statevector budget");
+    const std::string who = "serving: request " + std::to_string(r.id);
+    if (r.circuit) {
+      if (r.n_qubits != r.circuit->n_qubits())
+        throw Error(who + " has n_qubits " + std::to_string(r.n_qubits) + " but its circuit has " +
+                    std::to_string(r.circuit->n_qubits()));
+      if (r.n_qubits < 1 || r.n_qubits > kMaxQubits)
+        throw Error(who + " has an invalid qubit count (1.." + std::to_string(kMaxQubits) + ")");
+      for (const GatePtr &g : r.circuit->gates())
+        if (!g->is_unitary())
+          throw Error(who + " contains a noisy gate; serving takes unitary circuits");
+    } else if (r.n_qubits < 2 || r.n_qubits > kMaxQubits || r.depth < 1) {
+      throw Error(who + " has an invalid shape");
+    }
+    if (request_bytes(r, cfg_.f64_shadow && r.precision == CAST_PRECISION_F32) > pool.budget)
+      throw Error(who + " needs more memory than the statevector budget");
   }
 
   // ── compile workers ──────────────────────────────────────────────────────
   struct Compiled {
-    std::vector<void *> plans;
+    std::vector<void *> plans, shadow_plans;
     std::string error;
   };
   std::vector<Compiled> compiled(N);
@@ -249,28 +297,41 @@ std::vector<ServeResult> Server::run(const std::vector<ServeRequest> &reqs, Serv
       }
       Compiled c;
       try {
-        CircuitGraph g = CircuitGraph::from_circuit(random_layer_circuit(r.n_qubits, r.depth, r.seed));
+        CircuitGraph g = CircuitGraph::from_circuit(
+            r.circuit ? *r.circuit : random_layer_circuit(r.n_qubits, r.depth, r.seed));
         optimize(g, FusionConfig::size_only(cfg_.fusion_max));
-        const HipKernelGenSpec spec =
-            r.precision == CAST_PRECISION_F32 ? HipKernelGenSpec::f32() : HipKernelGenSpec::f64();
-        cast_hip_kernel_gen_spec_t cs{};
-        cs.precision = spec.precision;
-        cs.ztol = spec.ztol;
-        cs.otol = spec.otol;
-        cs.variant = CAST_HIP_VARIANT_AUTO;
-        char e[kErr];
-        for (const GatePtr &gp : g.gates_in_row_order()) {
-          const auto &m = gp->matrix().data();
-          std::vector<cast_complex64_t> cm(m.size());
-          for (size_t k = 0; k < m.size(); ++k)
-            cm[k] = {m[k].real(), m[k].imag()};
-          void *plan = nullptr;
-          if (cast_hip_gate_plan_create(&cs, cm.data(), cm.size(), gp->qubits().data(),
-                                        gp->qubits().size(), &plan, e, kErr) != 0)
-            throw Error(e);
-          c.plans.push_back(plan);
-        }
+        const std::vector<GatePtr> fused = g.gates_in_row_order();
+        auto compile = [&](cast_precision_t prec, std::vector<void *> &out) {
+          const HipKernelGenSpec spec =
+              prec == CAST_PRECISION_F32 ? HipKernelGenSpec::f32() : HipKernelGenSpec::f64();
+          cast_hip_kernel_gen_spec_t cs{};
+          cs.precision = spec.precision;
+          cs.ztol = spec.ztol;
+          cs.otol = spec.otol;
+          cs.variant = CAST_HIP_VARIANT_AUTO;
+          char e[kErr];
+          for (const GatePtr &gp : fused) {
+            const auto &m = gp->matrix().data();
+            std::vector<cast_complex64_t> cm(m.size());
+            for (size_t k = 0; k < m.size(); ++k)
+              cm[k] = {m[k].real(), m[k].imag()};
+            void *plan = nullptr;
+            if (cast_hip_gate_plan_create(&cs, cm.data(), cm.size(), gp->qubits().data(),
+                                          gp->qubits().size(), &plan, e, kErr) != 0)
+              throw Error(e);
+            out.push_back(plan);
+          }
+        };
+        compile(r.precision, c.plans);
+        if (cfg_.f64_shadow && r.precision == CAST_PRECISION_F32)
+          compile(CAST_PRECISION_F64, c.shadow_plans);
       } catch (const std::exception &ex) {
+        for (void *p : c.plans)
+          cast_hip_gate_plan_destroy(p);
+        for (void *p : c.shadow_plans)
+          cast_hip_gate_plan_destroy(p);
+        c.plans.clear();
+        c.shadow_plans.clear();
         c.error = ex.what();
       }
       const double tc = now_s() - t0;
@@ -296,7 +357,16 @@ std::vector<ServeResult> Server::run(const std::vector<ServeRequest> &reqs, Serv
     const ServeRequest &r = reqs[s.req];
     o.done_s = now_s() - t0;
     o.norm_sq = s.h_out[0];
-    o.zz.assign(s.h_out + 1, s.h_out + r.n_qubits);
+    const uint32_t np = r.n_qubits - 1;
+    o.zz.clear();
+    for (uint32_t c = 0; c * kPairs < np; ++c) {
+      const double *h = s.h_out + c * (1 + kPairs) + 1;
+      o.zz.insert(o.zz.end(), h, h + std::min(kPairs, np - c * kPairs));
+    }
+    if (s.shadow) {
+      const double re = s.h_out[kOutInner], im = s.h_out[kOutInner + 1];
+      o.fidelity_vs_f64 = (re * re + im * im) / (s.h_out[0] * s.h_out[kOutShadowNorm]);
+    }
     if (cfg_.keep_states) {
       const size_t n_el = size_t(2) << r.n_qubits;
       o.state.resize(n_el);
@@ -309,22 +379,30 @@ std::vector<ServeResult> Server::run(const std::vector<ServeRequest> &reqs, Serv
     }
     for (void *p : s.plans)
       cast_hip_gate_plan_destroy(p);
+    for (void *p : s.shadow_plans)
+      cast_hip_gate_plan_destroy(p);
     s.plans.clear();
+    s.shadow_plans.clear();
     pool.release(s.sv, s.sv_bytes);
     s.sv = 0;
+    s.shadow = 0;
     s.busy = false;
     --in_flight;
     ++done;
   };
-  auto admit = [&](Slot &s, size_t i, std::vector<void *> plans) {
+  auto admit = [&](Slot &s, size_t i, Compiled c) {
     const ServeRequest &r = reqs[i];
     const size_t b = sv_bytes_of(r.n_qubits, r.precision);
-    s.sv = pool.acquire(b);
+    const bool shadow = !c.shadow_plans.empty();
+    const size_t held = request_bytes(r, shadow);
+    s.sv = pool.acquire(held);
     if (!s.sv)
       throw Error("serving: statevector pool refused an admitted request");
-    s.sv_bytes = b;
+    s.sv_bytes = held;
+    s.shadow = shadow ? s.sv + round_up_256(b) : 0;
     s.req = i;
-    s.plans = std::move(plans);
+    s.plans = std::move(c.plans);
+    s.shadow_plans = std::move(c.shadow_plans);
     s.busy = true;
     ++in_flight;
     max_in_flight = std::max(max_in_flight, in_flight);
@@ -341,16 +419,36 @@ std::vector<ServeResult> Server::run(const std::vector<ServeRequest> &reqs, Serv
              err);
     for (void *p : s.plans)
       throw_if(cast_hip_gate_apply(p, s.stream, s.sv, r.n_qubits, err, kErr), err);
-    uint32_t pairs[2 * kPairs];
+    // Norm and neighbour pairs in chunks of kPairs (the scratch is reused in
+    // stream order).
     const uint32_t np = r.n_qubits - 1;
-    for (uint32_t j = 0; j < np; ++j) {
-      pairs[2 * j] = j;
-      pairs[2 * j + 1] = j + 1;
+    uint32_t pairs[2 * kPairs];
+    for (uint32_t c = 0; c == 0 || c * kPairs < np; ++c) {
+      const uint32_t m = std::min(kPairs, np - c * kPairs);
+      for (uint32_t j = 0; j < m; ++j) {
+        pairs[2 * j] = c * kPairs + j;
+        pairs[2 * j + 1] = c * kPairs + j + 1;
+      }
+      throw_if(cast_hip_observe_async(s.sv, r.n_qubits, prec, pairs, m,
+                                      s.out + c * (1 + kPairs) * sizeof(double), s.scratch,
+                                      s.stream, err, kErr),
+               err);
     }
-    throw_if(cast_hip_observe_async(s.sv, r.n_qubits, prec, pairs, np, s.out, s.scratch, s.stream,
-                                    err, kErr),
-             err);
-    throw_if(cast_hip_memcpy_dtoh_async(s.h_out, s.out, (1 + np) * sizeof(double), s.stream, err,
+    if (s.shadow) {
+      const size_t n_el = size_t(2) << r.n_qubits;
+      throw_if(cast_hip_sv_zero_async(s.shadow, n_el, 1, s.stream, err, kErr), err);
+      for (void *p : s.shadow_plans)
+        throw_if(cast_hip_gate_apply(p, s.stream, s.shadow, r.n_qubits, err, kErr), err);
+      throw_if(cast_hip_inner_product_async(s.sv, prec, s.shadow, 1, r.n_qubits,
+                                            s.out + kOutInner * sizeof(double), s.scratch,
+                                            s.stream, err, kErr),
+               err);
+      throw_if(cast_hip_observe_async(s.shadow, r.n_qubits, 1, pairs, 0,
+                                      s.out + kOutShadowNorm * sizeof(double), s.scratch,
+                                      s.stream, err, kErr),
+               err);
+    }
+    throw_if(cast_hip_memcpy_dtoh_async(s.h_out, s.out, kOut * sizeof(double), s.stream, err,
                                         kErr),
              err);
     if (cfg_.keep_states) {
@@ -385,7 +483,7 @@ std::vector<ServeResult> Server::run(const std::vector<ServeRequest> &reqs, Serv
         // Oldest compiled request that fits; younger ones may pass an older
         // one only when the older one's statevector does not fit yet.
         size_t pick = SIZE_MAX;
-        std::vector<void *> plans;
+        Compiled plans;
         {
           std::lock_guard<std::mutex> lock(mu);
           for (auto it = ready.begin(); it != ready.end(); ++it) {
@@ -393,9 +491,9 @@ std::vector<ServeResult> Server::run(const std::vector<ServeRequest> &reqs, Serv
             if (!compiled[i].error.empty())
               throw Error("serving: compiling request " + std::to_string(reqs[i].id) + ": " +
                           compiled[i].error);
-            if (pool.fits(sv_bytes_of(reqs[i].n_qubits, reqs[i].precision))) {
+            if (pool.fits(request_bytes(reqs[i], !compiled[i].shadow_plans.empty()))) {
               pick = i;
-              plans = std::move(compiled[i].plans);
+              plans = std::move(compiled[i]);
               ready.erase(it);
               break;
             }
@@ -424,14 +522,20 @@ std::vector<ServeResult> Server::run(const std::vector<ServeRequest> &reqs, Serv
       if (s->busy) {
         for (void *p : s->plans)
           cast_hip_gate_plan_destroy(p);
+        for (void *p : s->shadow_plans)
+          cast_hip_gate_plan_destroy(p);
         s->plans.clear();
+        s->shadow_plans.clear();
         pool.release(s->sv, s->sv_bytes);
         s->busy = false;
       }
     }
-    for (auto &c : compiled)
+    for (auto &c : compiled) {
       for (void *p : c.plans)
         cast_hip_gate_plan_destroy(p);
+      for (void *p : c.shadow_plans)
+        cast_hip_gate_plan_destroy(p);
+    }
     throw Error(failure);
   }
   const double wall = now_s() - t0;

@@ -20,6 +20,7 @@
 #include "hip_backend.hpp"
 
 #include <cstdint>
+#include <memory>
 #include <vector>
 
 namespace cast::hip {
@@ -28,10 +29,18 @@ struct ServeRequest {
   uint32_t id = 0;
   uint32_t n_qubits = 0;
   uint32_t depth = 0;
-  uint64_t seed = 0; // random_layer_circuit(n_qubits, depth, seed)
+  uint64_t seed = 0; // random_layer_circuit(n_qubits, depth, seed) when circuit is null
   cast_precision_t precision = CAST_PRECISION_F64;
   double arrival_s = 0.0; // offset from the start of the run
+  // The circuit to serve (any unitary circuit, e.g. parsed from OpenQASM);
+  // null = the seeded random-layer circuit above.  n_qubits must equal
+  // circuit->n_qubits().
+  std::shared_ptr<const QuantumCircuit> circuit;
 };
+
+// A request that serves `circuit` (n_qubits taken from it).
+ServeRequest circuit_request(uint32_t id, std::shared_ptr<const QuantumCircuit> circuit,
+                             cast_precision_t precision, double arrival_s);
 
 // `count` requests with exponential inter-arrival gaps of mean 1/rate,
 // n ~ U{n_min..n_max}, depth ~ U{depth_min..depth_max}, f32 with
@@ -42,7 +51,8 @@ std::vector<ServeRequest> poisson_requests(uint32_t count, double rate, uint32_t
                                            uint64_t seed);
 
 struct ServeConfig {
-  uint32_t streams = 8;          // concurrent requests on the device
+  uint32_t streams = 4;          // concurrent requests on the device (one per
+                                 // hardware queue of the MI355X process)
   uint32_t compile_threads = 8;  // host threads compiling arrived requests
   uint32_t fusion_max = 6;       // size-only fusion bound (1..6); k = 6 runs on
                                  // matrix cores and fewer passes serve more
@@ -51,6 +61,12 @@ struct ServeConfig {
   uint64_t reserve_bytes = uint64_t(8) << 30;
   bool realtime = true;          // release requests at their arrival times
   bool keep_states = false;      // copy every final statevector to the host
+  // Checking mode: every F32 request also runs its fused circuit at F64 on a
+  // second resident statevector, queued on the same stream, and the device
+  // computes the fidelity |<f32|f64>|^2 / (<f32|f32><f64|f64>) in f64.  It
+  // doubles the memory and adds the F64 work of each F32 request, so it is
+  // for accuracy runs, not for latency figures.
+  bool f64_shadow = false;
 };
 
 struct ServeResult {
@@ -59,6 +75,7 @@ struct ServeResult {
   double arrival_s = 0.0, compiled_s = 0.0, admit_s = 0.0, done_s = 0.0;
   double norm_sq = 0.0;
   std::vector<double> zz;    // <Z_i Z_i+1> sums, i = 0 .. n-2 (unnormalised)
+  double fidelity_vs_f64 = -1.0; // f64_shadow runs, F32 requests; -1 otherwise
   std::vector<double> state; // interleaved f64, when keep_states
 };
 

@@ -231,10 +231,12 @@ int cast_hip_gate_plan_family(const void *plan, uint32_t sv_n_qubits, uint32_t *
 int cast_hip_gate_apply(const void *plan, void *stream, uint64_t sv_dptr, uint32_t sv_n_qubits,
                         char *err_buf, size_t err_buf_len);
 
-/* Enqueue n_items independent (plan, statevector) pairs as ONE launch: item i
- * applies plans[i] to svs[i] (2^sv_n_qubits[i] amplitudes).  All plans must
- * share precision and k.  Used by batched trajectories and by serving.
- * No reference counterpart (the reference launches one kernel per apply). */
+/* Enqueue n_items independent (plan, statevector) pairs as ONE launch (64
+ * items per launch, more in several launches): item i applies plans[i] to
+ * svs[i] (2^sv_n_qubits[i] amplitudes, sv_n_qubits[i] <= 40) with the direct
+ * kernel.  All plans must share precision and k.  For callers that hold many
+ * small statevectors (one launch instead of one per statevector).  No
+ * reference counterpart (the reference launches one kernel per apply). */
 int cast_hip_gate_apply_batch(const void *const *plans, const uint64_t *svs,
                               const uint32_t *sv_n_qubits, size_t n_items, void *stream,
                               char *err_buf, size_t err_buf_len);
@@ -285,6 +287,17 @@ int cast_hip_dm_populations(uint64_t sv_dptr, uint32_t n_physical, uint8_t preci
 int cast_hip_inner_product(uint64_t a, uint8_t precision_a, uint64_t b, uint8_t precision_b,
                            uint32_t sv_n_qubits, double *out_re_im, void *stream,
                            char *err_buf, size_t err_buf_len);
+
+/* The same queued on `stream` (returns without waiting): device out[0..1] =
+ * {Re, Im} of <a|b>, using device `scratch` of
+ * cast_hip_inner_product_scratch_bytes() bytes that no other queued work uses
+ * until this completes.  Deterministic (fixed reduction order).  No reference
+ * counterpart (serving's f32-against-f64 check runs it between requests
+ * without waiting for the device). */
+size_t cast_hip_inner_product_scratch_bytes(void);
+int cast_hip_inner_product_async(uint64_t a, uint8_t precision_a, uint64_t b, uint8_t precision_b,
+                                 uint32_t sv_n_qubits, uint64_t out, uint64_t scratch,
+                                 void *stream, char *err_buf, size_t err_buf_len);
 
 /* ── Batched trajectories in one persistent launch ───────────────────────────
  * No reference counterpart (the reference branches deterministic ensembles
@@ -342,10 +355,11 @@ typedef struct cast_hip_traj_launch_t {
 } cast_hip_traj_launch_t;
 
 /* Enqueue the persistent launch on `stream` (returns without waiting).  A
- * gate descriptor that is invalid on the device (k outside 1..4, a qubit
- * >= n, a matrix outside the pool, a checkpoint outside the table) is
- * skipped and sets the error flag (second uint32_t of `work`), which the
- * caller checks after the stream completes. */
+ * gate descriptor that is invalid on the device (k outside 1..4 at F64 or
+ * 1..6 at F32, k = 6 on fewer than 12 qubits, a qubit >= n, a matrix outside
+ * the pool, a checkpoint outside the table) is skipped and sets the error
+ * flag (second uint32_t of `work`), which the caller checks after the stream
+ * completes. */
 int cast_hip_traj_run(const cast_hip_traj_launch_t *args, void *stream, char *err_buf,
                       size_t err_buf_len);
 

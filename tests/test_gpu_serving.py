@@ -11,25 +11,28 @@ import pytest
 
 import cast_simulator_amd as cs
 import oracle
-from helpers import rel_l2
+from helpers import fidelity, rel_l2
 
 pytestmark = pytest.mark.gpu
 
 
-def serve(reqs, streams, threads=4, realtime=False, keep=True, budget=0):
+def serve(reqs, streams, threads=4, realtime=False, keep=True, budget=0, shadow=False):
     cfg = cs.ServeConfig()
     cfg.streams, cfg.compile_threads, cfg.realtime, cfg.keep_states = streams, threads, realtime, keep
     cfg.budget_bytes = budget
+    cfg.f64_shadow = shadow
     print(f"[serve] {len(reqs)} requests, streams={streams}, realtime={realtime}", flush=True)
     res, st = cs.Server(cfg).run(reqs)
     return {r.id: r for r in res}, st
 
 
-def oracle_state(r):
-    g = cs.CircuitGraph.from_circuit(cs.random_layer_circuit(r.n_qubits, r.depth, r.seed))
+def oracle_state(r, circuit=None):
+    """The oracle's state of a request: its unfused circuit on |0...0>."""
+    if circuit is None:
+        circuit = cs.random_layer_circuit(r.n_qubits, r.depth, r.seed)
     sv = np.zeros(2 << r.n_qubits)
     sv[0] = 1.0
-    for gate in g.gates_in_row_order():
+    for gate in circuit.gates():
         oracle.apply_gate(sv, np.asarray(gate.matrix()), list(gate.qubits()))
     return sv
 
@@ -81,3 +84,99 @@ def test_requests_larger_than_the_budget_are_rejected():
     reqs = [cs.ServeRequest(0, 18, 4, 1, cs.Precision.F64, 0.0)]
     with pytest.raises(cs.CastError, match="budget"):
         serve(reqs, streams=2, keep=False, budget=(2 << 17) * 8)
+
+
+QASM = """OPENQASM 2.0;
+include "qelib1.inc";
+qreg q[7];
+h q[0];
+cx q[0],q[1];
+rz(pi/4) q[1];
+ccx q[0],q[1],q[2];
+u3(0.3,0.2,0.1) q[5];
+swap q[2],q[6];
+cp(pi/8) q[3],q[5];
+ry(-0.7) q[4];
+cz q[4],q[6];
+"""
+
+
+def test_circuit_and_qasm_requests_match_the_oracle():
+    """Requests carry any unitary circuit (a QFT, an OpenQASM program) beside
+    the generated random-layer ones; each is fused, served in the mix and
+    matches the oracle run of its unfused circuit."""
+    qft = cs.QuantumCircuit.qft(10)
+    prog = cs.QuantumCircuit.from_qasm(QASM)
+    reqs = [cs.ServeRequest.from_circuit(0, qft, cs.Precision.F64),
+            cs.ServeRequest.from_circuit(1, qft, cs.Precision.F32, 0.001),
+            cs.ServeRequest.from_qasm(2, QASM, cs.Precision.F64, 0.002),
+            cs.ServeRequest.from_qasm(3, QASM, cs.Precision.F32, 0.003)]
+    extra = cs.poisson_requests(12, 1000.0, 6, 14, 3, 12, 0.5, 21)
+    for r in extra:
+        r.id += 10
+    circuits = {0: qft, 1: qft, 2: prog, 3: prog}
+    assert reqs[2].n_qubits == 7 and reqs[0].has_circuit and not extra[0].has_circuit
+    res, st = serve(reqs + extra, streams=4, threads=4)
+    assert st["max_in_flight"] <= 4
+    for r in reqs + extra:
+        o = res[r.id]
+        want = oracle_state(r, circuits.get(r.id))
+        tol = 1e-12 if r.precision == cs.Precision.F64 else 1e-5
+        assert rel_l2(want, o.state()) < tol, f"request {r.id}"
+        assert len(o.zz) == r.n_qubits - 1
+
+
+def test_requests_with_noise_or_a_wrong_qubit_count_are_rejected():
+    c = cs.QuantumCircuit(3)
+    c.add(cs.QuantumGate.h(0))
+    c.add(cs.QuantumGate.depolarizing(1, 0.1))
+    with pytest.raises(cs.CastError, match="noisy"):
+        serve([cs.ServeRequest.from_circuit(0, c, cs.Precision.F64)], streams=1)
+    r = cs.ServeRequest.from_qasm(0, QASM, cs.Precision.F64)
+    r.n_qubits = 9
+    with pytest.raises(cs.CastError, match="n_qubits"):
+        serve([r], streams=1)
+
+
+def test_f64_shadow_gives_the_fidelity_of_every_f32_request():
+    """Checking mode: each F32 request also runs at F64 on the device; the
+    device fidelity equals the host fidelity of the F32 result against the
+    oracle's f64 state, is >= 1 - 1e-6, and the F32 results are bit-identical
+    to a run without the shadow."""
+    reqs = cs.poisson_requests(24, 1000.0, 8, 16, 3, 20, 0.5, 13)
+    plain, _ = serve(reqs, streams=4)
+    res, st = serve(reqs, streams=4, shadow=True)
+    n32 = 0
+    for r in reqs:
+        o = res[r.id]
+        assert np.array_equal(o.state(), plain[r.id].state()), f"request {r.id}"
+        if r.precision == cs.Precision.F32:
+            n32 += 1
+            f_host = fidelity(oracle_state(r), o.state())
+            assert abs(o.fidelity_vs_f64 - f_host) < 1e-9, (o.fidelity_vs_f64, f_host)
+            assert o.fidelity_vs_f64 >= 1.0 - 1e-6
+        else:
+            assert o.fidelity_vs_f64 == -1.0
+    assert n32 >= 6
+
+
+def test_34_qubit_f32_request_reduces_every_neighbour_pair():
+    """A 34-qubit f32 request (128 GiB) has 33 neighbour pairs, reduced in two
+    chunks.  RY(theta_i) on every qubit gives <Z_i Z_i+1> =
+    cos(theta_i) cos(theta_i+1) exactly."""
+    n = 34
+    from cast_simulator_amd import abi
+
+    free = abi.free_memory()[0]
+    if free < (2 << n) * 4 + (10 << 30):
+        pytest.skip(f"needs 138 GiB of device memory, {free / 2**30:.0f} GiB free")
+    theta = 0.1 + 0.045 * np.arange(n)
+    c = cs.QuantumCircuit(n)
+    for q in range(n):
+        c.add(cs.QuantumGate.ry(float(theta[q]), q))
+    res, st = serve([cs.ServeRequest.from_circuit(0, c, cs.Precision.F32)], streams=1, keep=False)
+    o = res[0]
+    want = np.cos(theta[:-1]) * np.cos(theta[1:])
+    assert len(o.zz) == n - 1
+    assert abs(o.norm_sq - 1.0) < 1e-5
+    assert np.allclose(np.asarray(o.zz) / o.norm_sq, want, rtol=0, atol=1e-5)

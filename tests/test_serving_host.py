@@ -35,3 +35,23 @@ def test_poisson_requests_are_seeded_and_in_range():
 def test_poisson_requests_reject_bad_parameters(args):
     with pytest.raises(cs.CastError):
         cs.poisson_requests(*args)
+
+
+def test_circuit_requests_take_their_qubit_count_from_the_circuit():
+    qasm = 'OPENQASM 2.0;\ninclude "qelib1.inc";\nqreg q[9];\nh q[0];\ncx q[0],q[5];\n'
+    a = cs.ServeRequest.from_qasm(3, qasm, cs.Precision.F32, 0.25)
+    assert (a.id, a.n_qubits, a.depth, a.precision, a.arrival_s) == (3, 6, 2, cs.Precision.F32, 0.25)
+    assert a.has_circuit
+    b = cs.ServeRequest.from_circuit(4, cs.QuantumCircuit.qft(7), cs.Precision.F64)
+    assert b.n_qubits == 7 and b.depth == 7 + 21 + 3 and b.has_circuit
+    c = b.with_precision(cs.Precision.F32)
+    assert c.precision == cs.Precision.F32 and c.has_circuit and b.precision == cs.Precision.F64
+    g = cs.ServeRequest(0, 12, 10, 1, cs.Precision.F64)
+    assert not g.has_circuit
+    with pytest.raises(cs.CastError):
+        cs.ServeRequest.from_qasm(0, "qreg q[2];\nfoo q[0];\n", cs.Precision.F64)
+
+
+def test_serve_config_defaults_to_one_stream_per_hardware_queue():
+    cfg = cs.ServeConfig()
+    assert cfg.streams == 4 and cfg.f64_shadow is False
