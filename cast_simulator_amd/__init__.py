@@ -67,6 +67,7 @@ from ._host import (  # noqa: E402,F401
     adaptive_sweep,
     apply_size_two_fusion,
     batch_sample,
+    clean_branch,
     compute_bw_slope,
     expand_matrix,
     fit_from_samples,

@@ -528,6 +528,7 @@ PYBIND11_MODULE(_host, m) {
   py::class_<LayeredCircuit>(m, "LayeredCircuit")
       .def_static("random_layers", &LayeredCircuit::random_layers, py::arg("n"),
                   py::arg("depth"), py::arg("seed"))
+      .def_static("from_circuit", &LayeredCircuit::from_circuit, py::arg("circuit"))
       .def_readonly("n_qubits", &LayeredCircuit::n_qubits)
       .def_readonly("layer_start", &LayeredCircuit::layer_start)
       .def("n_layers", &LayeredCircuit::n_layers)
@@ -553,12 +554,19 @@ PYBIND11_MODULE(_host, m) {
       .def_readwrite("max_depth", &NoiseSpec::max_depth)
       .def_readwrite("seed", &NoiseSpec::seed);
 
+  // Events as tuples (gate, qubit, pauli) with pauli 1..3 for a Pauli on
+  // `qubit` after the gate, or (gate, branch, 0) for a noisy gate that took
+  // noise branch `branch` instead of its most likely one.
   py::class_<TrajectorySample>(m, "TrajectorySample")
       .def(py::init([](uint32_t depth, const std::vector<std::tuple<uint32_t, uint32_t, uint8_t>> &e) {
              TrajectorySample s;
              s.depth = depth;
-             for (const auto &[g, q, p] : e)
-               s.errors.push_back({g, q, p});
+             for (const auto &[g, q, p] : e) {
+               if (p == 0)
+                 s.errors.push_back({g, 0, 0, q});
+               else
+                 s.errors.push_back({g, q, p, 0});
+             }
              return s;
            }),
            py::arg("depth"), py::arg("errors"))
@@ -566,9 +574,14 @@ PYBIND11_MODULE(_host, m) {
       .def_property_readonly("errors", [](const TrajectorySample &s) {
         std::vector<std::tuple<uint32_t, uint32_t, uint8_t>> out;
         for (const auto &e : s.errors)
-          out.emplace_back(e.gate, e.qubit, e.pauli);
+          out.emplace_back(e.gate, e.pauli == 0 ? e.branch : e.qubit, e.pauli);
         return out;
       });
+  m.def("clean_branch", [](const QuantumGate &g) {
+    if (!g.noise_model())
+      throw Error("clean_branch: the gate has no noise model");
+    return clean_branch(*g.noise_model());
+  });
 
   m.def("sample_trajectories", &sample_trajectories, py::arg("circuit"), py::arg("noise"),
         py::arg("count"), py::arg("first") = 0);

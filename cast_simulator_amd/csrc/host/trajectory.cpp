@@ -61,6 +61,42 @@ LayeredCircuit LayeredCircuit::random_layers(uint32_t n, uint32_t depth, uint64_
   return c;
 }
 
+LayeredCircuit LayeredCircuit::from_circuit(const QuantumCircuit &qc) {
+  LayeredCircuit c;
+  c.n_qubits = qc.n_qubits();
+  std::vector<uint32_t> last(c.n_qubits, 0); // per qubit: layers already holding it
+  std::vector<std::vector<GatePtr>> layers;
+  for (const GatePtr &g : qc.gates()) {
+    uint32_t l = 0;
+    for (uint32_t q : g->qubits()) {
+      if (q >= c.n_qubits)
+        throw Error("from_circuit: gate qubit outside the circuit");
+      l = std::max(l, last[q]);
+    }
+    for (uint32_t q : g->qubits())
+      last[q] = l + 1;
+    if (layers.size() <= l)
+      layers.resize(l + 1);
+    layers[l].push_back(g);
+  }
+  for (const auto &layer : layers) {
+    for (const GatePtr &g : layer)
+      c.gates.push_back(g);
+    c.layer_start.push_back(uint32_t(c.gates.size()));
+  }
+  return c;
+}
+
+uint32_t clean_branch(const NoiseModel &m) {
+  if (m.branches.empty())
+    throw Error("noise model without branches");
+  uint32_t best = 0;
+  for (uint32_t b = 1; b < m.branches.size(); ++b)
+    if (m.branches[b].first > m.branches[best].first)
+      best = b;
+  return best;
+}
+
 std::vector<TrajectorySample> sample_trajectories(const LayeredCircuit &c, const NoiseSpec &ns,
                                                   uint32_t count, uint32_t first) {
   if (ns.min_depth < 1 || ns.min_depth > ns.max_depth || ns.max_depth > c.n_layers())
@@ -75,22 +111,78 @@ std::vector<TrajectorySample> sample_trajectories(const LayeredCircuit &c, const
     TrajectorySample &s = out[i];
     s.depth = ns.min_depth + uint32_t(rng.below(ns.max_depth - ns.min_depth + 1));
     const uint32_t end = c.layer_start[s.depth];
-    for (uint32_t g = 0; g < end; ++g)
-      for (uint32_t q : c.gates[g]->qubits())
-        if (rng.uniform() < ns.p)
-          s.errors.push_back({g, q, uint8_t(1 + rng.below(3))});
+    for (uint32_t g = 0; g < end; ++g) {
+      if (const auto &nm = c.gates[g]->noise_model()) {
+        // Inverse CDF over the branch probabilities (the last branch takes
+        // any rounding remainder).
+        const double u = rng.uniform();
+        const uint32_t nb = uint32_t(nm->branches.size());
+        uint32_t b = 0;
+        double acc = nm->branches[0].first;
+        while (b + 1 < nb && u >= acc)
+          acc += nm->branches[++b].first;
+        if (b != clean_branch(*nm))
+          s.errors.push_back({g, 0, 0, b});
+      }
+      if (ns.p > 0.0)
+        for (uint32_t q : c.gates[g]->qubits())
+          if (rng.uniform() < ns.p)
+            s.errors.push_back({g, q, uint8_t(1 + rng.below(3)), 0});
+    }
   }
   return out;
 }
+
+namespace {
+// Operator source gate g applies in a trajectory whose events at g are
+// ev[0..n): the clean or sampled branch (or the unitary), then the Paulis.
+ComplexSquareMatrix trajectory_operator(const QuantumGate &g, const NoiseEvent *ev, size_t n) {
+  ComplexSquareMatrix m = g.matrix();
+  size_t i = 0;
+  if (const auto &nm = g.noise_model()) {
+    uint32_t b = clean_branch(*nm);
+    if (n > 0 && ev[0].pauli == 0) {
+      b = ev[0].branch;
+      i = 1;
+    }
+    if (b >= nm->branches.size())
+      throw Error("noise event names branch " + std::to_string(b) + " of a gate with " +
+                  std::to_string(nm->branches.size()));
+    m = nm->branches[b].second;
+  }
+  for (; i < n; ++i) {
+    if (ev[i].pauli < 1 || ev[i].pauli > 3)
+      throw Error("a gate's branch event must come before its Pauli events");
+    m = expand_matrix(pauli_matrix(ev[i].pauli), {ev[i].qubit}, g.qubits()).matmul(m);
+  }
+  return m;
+}
+} // namespace
 
 QuantumCircuit trajectory_circuit(const LayeredCircuit &c, const TrajectorySample &s) {
   QuantumCircuit qc(c.n_qubits);
   size_t e = 0;
   const uint32_t end = c.layer_start.at(s.depth);
   for (uint32_t g = 0; g < end; ++g) {
-    qc.add(c.gates[g]);
-    for (; e < s.errors.size() && s.errors[e].gate == g; ++e)
+    const QuantumGate &src = *c.gates[g];
+    size_t e1 = e;
+    while (e1 < s.errors.size() && s.errors[e1].gate == g)
+      ++e1;
+    if (src.noise_model()) {
+      // the branch as a plain gate, then the Paulis
+      const bool has_branch = e1 > e && s.errors[e].pauli == 0;
+      qc.add(QuantumGate(trajectory_operator(src, s.errors.data() + e, has_branch ? 1 : 0),
+                         src.qubits()));
+      if (has_branch)
+        ++e;
+    } else {
+      qc.add(c.gates[g]);
+    }
+    for (; e < e1; ++e) {
+      if (s.errors[e].pauli < 1 || s.errors[e].pauli > 3)
+        throw Error("trajectory_circuit: a branch event on a unitary gate or after a Pauli");
       qc.add(QuantumGate(pauli_matrix(s.errors[e].pauli), {s.errors[e].qubit}));
+    }
   }
   if (e != s.errors.size())
     throw Error("trajectory_circuit: errors out of order or beyond the trajectory's depth");
@@ -129,9 +221,12 @@ TrajectoryEngine::TrajectoryEngine(LayeredCircuit circuit, cast_precision_t prec
     if (pairs_[i] >= circ_.n_qubits || pairs_[i + 1] >= circ_.n_qubits ||
         pairs_[i] == pairs_[i + 1])
       throw Error("TrajectoryEngine: pair qubits must be distinct and below n");
-  for (const auto &g : circ_.gates)
-    if (!g->is_unitary() || g->n_qubits() > 4)
-      throw Error("TrajectoryEngine: source gates must be unitary and act on <= 4 qubits");
+  clean_.reserve(circ_.gates.size());
+  for (const auto &g : circ_.gates) {
+    if (g->n_qubits() > 4)
+      throw Error("TrajectoryEngine: source gates must act on <= 4 qubits");
+    clean_.push_back(trajectory_operator(*g, nullptr, 0));
+  }
   mgr_.set_autotune(false);
 }
 
@@ -159,7 +254,7 @@ const TrajectoryEngine::Segment &TrajectoryEngine::segment(uint32_t kmax, uint32
     return it->second;
   QuantumCircuit qc(circ_.n_qubits);
   for (uint32_t g = circ_.layer_start[l0]; g < circ_.layer_start[l1]; ++g)
-    qc.add(circ_.gates[g]->with_origin({g}));
+    qc.add(QuantumGate(clean_[g], circ_.gates[g]->qubits()).with_origin({g}));
   CircuitGraph cg = CircuitGraph::from_circuit(qc);
   optimize(cg, FusionConfig::size_only(kmax));
   Segment seg;
@@ -186,31 +281,72 @@ const TrajectoryEngine::Segment &TrajectoryEngine::segment(uint32_t kmax, uint32
   return segs_.emplace(key, std::move(seg)).first->second;
 }
 
+ComplexSquareMatrix TrajectoryEngine::component(const FusedGate &f, uint32_t pos) const {
+  const uint32_t g = f.origin[pos];
+  return expand_matrix(clean_[g], circ_.gates[g]->qubits(), f.qubits);
+}
+
+// prefix(f, i) = C_i ... C_0 and suffix(f, i) = C_{m-1} ... C_i for the
+// clean components C of F (filled on demand; callers on several threads only
+// read entries filled before they start).
 const ComplexSquareMatrix &TrajectoryEngine::prefix(const FusedGate &f, uint32_t pos) const {
   while (f.prefix.size() <= pos) {
     const size_t i = f.prefix.size();
-    const QuantumGate &g = *circ_.gates[f.origin[i]];
-    ComplexSquareMatrix e = expand_matrix(g.matrix(), g.qubits(), f.qubits);
+    ComplexSquareMatrix e = component(f, uint32_t(i));
     f.prefix.push_back(i == 0 ? std::move(e) : e.matmul(f.prefix[i - 1]));
   }
   return f.prefix[pos];
 }
 
-// F with Paulis after some of its components: F' = F prod_e (B_e^dag P_e B_e),
-// errors taken by descending component position (B_e = product of the
-// components up to and including the erroneous one).
+const ComplexSquareMatrix &TrajectoryEngine::suffix(const FusedGate &f, uint32_t pos) const {
+  const uint32_t m = uint32_t(f.origin.size());
+  // f.suffix[j] = C_{m-1} ... C_{m-1-j}
+  while (f.suffix.size() < m - pos) {
+    const size_t j = f.suffix.size();
+    ComplexSquareMatrix e = component(f, uint32_t(m - 1 - j));
+    f.suffix.push_back(j == 0 ? std::move(e) : f.suffix[j - 1].matmul(e));
+  }
+  return f.suffix[m - 1 - pos];
+}
+
+// F with deviating components: F' = S X_last ... X_first P, where P =
+// prefix(first - 1), S = suffix(last + 1), and X_i is the trajectory's
+// operator for a component with events (its sampled branch, then its Paulis,
+// expanded to F's qubits) or the clean component between them.  `errs` holds
+// (component position, event) pairs in circuit order.
 ComplexSquareMatrix
 TrajectoryEngine::dirty_matrix(const FusedGate &f,
-                               std::vector<std::pair<uint32_t, PauliError>> errs) const {
+                               std::vector<std::pair<uint32_t, NoiseEvent>> errs) const {
   std::stable_sort(errs.begin(), errs.end(),
-                   [](const auto &a, const auto &b) { return a.first > b.first; });
-  ComplexSquareMatrix m = f.m;
-  for (const auto &[pos, e] : errs) {
-    const ComplexSquareMatrix &b = prefix(f, pos);
-    const ComplexSquareMatrix p = expand_matrix(pauli_matrix(e.pauli), {e.qubit}, f.qubits);
-    m = m.matmul(b.adjoint().matmul(p.matmul(b)));
+                   [](const auto &a, const auto &b) { return a.first < b.first; });
+  const uint32_t first = errs.front().first, last = errs.back().first;
+  const uint32_t m = uint32_t(f.origin.size());
+  ComplexSquareMatrix acc;
+  bool have = false;
+  if (first > 0) {
+    acc = prefix(f, first - 1);
+    have = true;
   }
-  return m;
+  std::vector<NoiseEvent> ev;
+  size_t e = 0;
+  for (uint32_t pos = first; pos <= last; ++pos) {
+    ev.clear();
+    for (; e < errs.size() && errs[e].first == pos; ++e)
+      ev.push_back(errs[e].second);
+    ComplexSquareMatrix x;
+    if (ev.empty()) {
+      x = component(f, pos);
+    } else {
+      const uint32_t g = f.origin[pos];
+      const QuantumGate &src = *circ_.gates[g];
+      x = expand_matrix(trajectory_operator(src, ev.data(), ev.size()), src.qubits(), f.qubits);
+    }
+    acc = have ? x.matmul(acc) : std::move(x);
+    have = true;
+  }
+  if (last + 1 < m)
+    acc = suffix(f, last + 1).matmul(acc);
+  return acc;
 }
 
 void TrajectoryEngine::ensure_checkpoints(uint32_t B) {
@@ -298,7 +434,7 @@ TrajectoryPlan TrajectoryEngine::plan(const std::vector<TrajectorySample> &sampl
   struct DirtyJob {
     uint32_t off;
     const FusedGate *f;
-    std::vector<std::pair<uint32_t, PauliError>> errs;
+    std::vector<std::pair<uint32_t, NoiseEvent>> errs;
   };
   std::vector<DirtyJob> jobs;
   auto slots_of = [](const ComplexSquareMatrix &m) {
@@ -327,10 +463,10 @@ TrajectoryPlan TrajectoryEngine::plan(const std::vector<TrajectorySample> &sampl
   };
   // Appends the fused gates of layers [l0, l1) with the given errors; clean
   // matrices of a segment sit consecutively in the pool.
-  auto push_segment = [&](uint32_t l0, uint32_t l1, const PauliError *e, size_t n_e) {
+  auto push_segment = [&](uint32_t l0, uint32_t l1, const NoiseEvent *e, size_t n_e) {
     const Segment &seg = segment(kmax, l0, l1);
     const uint32_t base = seg_offsets(seg);
-    std::map<uint32_t, std::vector<std::pair<uint32_t, PauliError>>> dirty;
+    std::map<uint32_t, std::vector<std::pair<uint32_t, NoiseEvent>>> dirty;
     for (size_t i = 0; i < n_e; ++i) {
       const auto w = seg.where.at(e[i].gate);
       dirty[w.first].push_back({w.second, e[i]});
@@ -342,9 +478,16 @@ TrajectoryPlan TrajectoryEngine::plan(const std::vector<TrajectorySample> &sampl
       if (it == dirty.end()) {
         push_gate(f, off);
       } else {
-        // Fill the component-product cache here: the threads only read it.
-        for (const auto &pe : it->second)
-          prefix(f, pe.first);
+        // Fill the component-product caches here: the threads only read them.
+        uint32_t lo = ~0u, hi = 0;
+        for (const auto &pe : it->second) {
+          lo = std::min(lo, pe.first);
+          hi = std::max(hi, pe.first);
+        }
+        if (lo > 0)
+          prefix(f, lo - 1);
+        if (hi + 1 < f.origin.size())
+          suffix(f, hi + 1);
         const uint32_t doff = reserve_matrix(f.m);
         jobs.push_back({doff, &f, std::move(it->second)});
         push_gate(f, doff);
@@ -605,16 +748,17 @@ TrajectoryRun TrajectoryEngine::run(const std::vector<TrajectorySample> &samples
   R.min_norm_sq = *std::min_element(R.norm_sq.begin(), R.norm_sq.end());
   R.max_norm_sq = *std::max_element(R.norm_sq.begin(), R.norm_sq.end());
   R.traj_zz.resize(samples.size());
-  double acc = 0.0;
+  double acc_zz = 0.0, acc_norm = 0.0;
   for (size_t t = 0; t < samples.size(); ++t) {
     const uint32_t k = R.task_of[t];
     double z = 0.0;
     for (uint32_t j = 0; j < NP; ++j)
       z += R.zz_sum[size_t(k) * NP + j];
     R.traj_zz[t] = NP ? z / (double(NP) * R.norm_sq[k]) : 0.0;
-    acc += R.traj_zz[t];
+    acc_zz += NP ? z / double(NP) : 0.0;
+    acc_norm += R.norm_sq[k];
   }
-  R.mean_zz = acc / double(samples.size());
+  R.mean_zz = acc_zz / acc_norm;
   R.total_s = now_s() - t_start;
   return R;
 }

@@ -57,11 +57,14 @@ def check_run(lc, S, prec, tuning, pairs, init=None, tol=None):
         P = len(pairs)
         assert abs(r.norm_sq[k] - ns) < 1e-12 * max(1.0, ns)
         assert np.allclose(r.zz_sum[k * P:(k + 1) * P], zz, rtol=0, atol=1e-12)
-    # host-side normalised means
+    # host-side f64 results: per-trajectory normalised values and the batch
+    # ratio sum_t (1/P) sum_j zz / sum_t norm
     P = len(pairs)
-    want_mean = np.mean([sum(r.zz_sum[r.task_of[t] * P:(r.task_of[t] + 1) * P]) /
-                         (P * r.norm_sq[r.task_of[t]]) for t in range(len(S))])
-    assert abs(r.mean_zz - want_mean) < 1e-12
+    ks = [r.task_of[t] for t in range(len(S))]
+    zz_t = np.array([sum(r.zz_sum[k * P:(k + 1) * P]) / P for k in ks])
+    nrm_t = np.array([r.norm_sq[k] for k in ks])
+    assert np.allclose(r.traj_zz, zz_t / nrm_t, rtol=0, atol=1e-12)
+    assert abs(r.mean_zz - zz_t.sum() / nrm_t.sum()) < 1e-12
     return r
 
 
@@ -302,3 +305,73 @@ def test_k6_needs_twelve_qubits():
     S = cs.sample_trajectories(lc, cs.NoiseSpec(0.05, 1, 4, 4), 4)
     with pytest.raises(cs.CastError, match="1..5"):
         cs.TrajectoryEngine(lc, F32, [0, 1]).run(S, cs.TrajectoryTuning(2, 6, 256, 2, 0))
+
+
+# ── circuits with their own noise models ────────────────────────────────────
+
+def noisy_layers(n, depth, seed, gamma, pxx):
+    """Random layers, a correlated XX branch (p = pxx) on every CZ, and
+    amplitude damping (non-unitary branches) on qubits 0 and n - 1 after
+    every layer; in moments."""
+    X = np.array([[0, 1], [1, 0]], dtype=complex)
+    a0 = np.array([[1, 0], [0, np.sqrt(1 - gamma)]], dtype=complex)
+    a1 = np.array([[0, np.sqrt(gamma)], [0, 0]], dtype=complex)
+    p1 = 0.3
+    qc = cs.QuantumCircuit(n)
+    per_layer = n + n // 2
+    for i, g in enumerate(cs.random_layer_circuit(n, depth, seed).gates()):
+        if g.n_qubits() == 2:
+            g = g.with_noise([(1 - pxx, np.eye(4)), (pxx, np.kron(X, X))])
+        qc.add(g)
+        if i % per_layer == per_layer - 1:
+            for q in (0, n - 1):
+                qc.add(cs.QuantumGate(np.eye(2), [q]).with_noise(
+                    [(1 - p1, a0 / np.sqrt(1 - p1)), (p1, a1 / np.sqrt(p1))]))
+    return cs.LayeredCircuit.from_circuit(qc)
+
+
+@pytest.mark.parametrize("prec,kmax", [(F64, 4), (F32, 4), (F32, 6)])
+def test_noise_model_tasks_match_oracle(prec, kmax):
+    """Non-unitary and correlated branches sampled per trajectory (plus
+    depolarizing Paulis): every dumped state against the oracle run of the
+    trajectory's explicit gate list; norms differ from 1 and are divided out
+    in f64."""
+    n = 12
+    lc = noisy_layers(n, 5, 61, gamma=0.3, pxx=0.1)
+    S = cs.sample_trajectories(lc, cs.NoiseSpec(0.01, 1, lc.n_layers(), 62), 32)
+    assert sum(1 for s in S for e in s.errors if e[2] == 0) > 10
+    init = np.asarray(cs.random_statevector(n, 63))
+    pairs = [(i, i + 1) for i in range(n - 1)]
+    r = check_run(lc, S, prec, cs.TrajectoryTuning(2, kmax, 256, 2, 0), pairs, init)
+    assert r.min_norm_sq < 0.99
+
+
+def test_noise_model_mean_matches_density_matrix():
+    """<(1/P) sum Z_i Z_i+1> over 4096 f64 trajectories of a 6-qubit noisy
+    circuit against the exact density-matrix value (every noisy gate lifted
+    to sum_b p_b K_b (x) conj(K_b), src/types/gate.rs:236-258, applied by the
+    oracle), within 3 sigma of the ratio estimator."""
+    n, count = 6, 4096
+    lc = noisy_layers(n, 4, 71, gamma=0.5, pxx=0.2)
+    D = lc.n_layers()
+    pairs = [(i, i + 1) for i in range(n - 1)]
+    v = np.zeros(2 << (2 * n))
+    v[0] = 1.0
+    for g in lc.gates():
+        sg = g.to_density_matrix_gate(n)
+        oracle.apply_gate(v, np.asarray(sg.matrix()), list(sg.qubits()))
+    rho = to_complex(v).reshape(1 << n, 1 << n).T
+    idx = np.arange(1 << n)
+    zzv = np.mean([1 - 2 * (((idx >> a) ^ (idx >> b)) & 1) for a, b in pairs], axis=0)
+    exact = float(np.real(np.sum(np.diag(rho) * zzv)))
+    S = cs.sample_trajectories(lc, cs.NoiseSpec(0.0, D, D, 72), count)
+    eng = cs.TrajectoryEngine(lc, F64, [q for p in pairs for q in p])
+    r = eng.run(S, cs.TrajectoryTuning(2, 4, 256, 2, 0))
+    P = len(pairs)
+    zz = np.array([sum(r.zz_sum[r.task_of[t] * P:(r.task_of[t] + 1) * P]) / P
+                   for t in range(count)])
+    nrm = np.array([r.norm_sq[r.task_of[t]] for t in range(count)])
+    sigma = np.std(zz - r.mean_zz * nrm, ddof=1) / np.sqrt(count) / nrm.mean()
+    print(f"[traj] noise-model mean {r.mean_zz:.6f} exact {exact:.6f} sigma {sigma:.2e}",
+          flush=True)
+    assert abs(r.mean_zz - exact) < 3 * sigma, (r.mean_zz, exact, sigma)

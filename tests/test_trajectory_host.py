@@ -282,3 +282,164 @@ def test_sampled_trajectories_converge_to_density_matrix():
     S0 = cs.sample_trajectories(lc, cs.NoiseSpec(0.0, D, D, 1), 3)
     sv = oracle_run(cs.trajectory_circuit(lc, S0[0]).gates(), z0)
     assert abs(float(np.sum(np.abs(to_complex(sv)) ** 2 * zzv)) - dm_zz_by_depth(lc, 0.0, pairs)[D]) < 1e-12
+
+
+# ── circuits with their own noise models (src/types/noise.rs:12-33) ─────────
+
+def amplitude_damping(q, gamma, p1):
+    """Amplitude damping as the reference's noise model: branches (p_b, A_b /
+    sqrt(p_b)) with Kraus operators A_0 = diag(1, sqrt(1 - gamma)), A_1 =
+    sqrt(gamma) |0><1|, so that sum_b p_b K_b rho K_b^dag is the channel.
+    Neither branch is unitary."""
+    a0 = np.array([[1, 0], [0, np.sqrt(1 - gamma)]], dtype=complex)
+    a1 = np.array([[0, np.sqrt(gamma)], [0, 0]], dtype=complex)
+    p0 = 1.0 - p1
+    return cs.QuantumGate(np.eye(2), [q]).with_noise(
+        [(p0, a0 / np.sqrt(p0)), (p1, a1 / np.sqrt(p1))])
+
+
+def noisy_circuit(n, depth, seed, gamma=0.2, pxx=0.1):
+    """Random layers with a correlated XX error on every CZ (branch 1, p =
+    pxx) and amplitude damping on qubit 0 and n - 1 after every layer."""
+    XX = np.kron(PAULI[1], PAULI[1])
+    base = cs.random_layer_circuit(n, depth, seed).gates()
+    qc = cs.QuantumCircuit(n)
+    per_layer = n + n // 2
+    for i, g in enumerate(base):
+        if g.n_qubits() == 2:
+            g = g.with_noise([(1 - pxx, np.eye(4)), (pxx, XX)])
+        qc.add(g)
+        if i % per_layer == per_layer - 1:
+            qc.add(amplitude_damping(0, gamma, 0.3))
+            qc.add(amplitude_damping(n - 1, gamma, 0.3))
+    return qc
+
+
+def test_from_circuit_layers_are_moments_and_keep_the_operator():
+    qc = noisy_circuit(5, 3, 12)
+    lc = cs.LayeredCircuit.from_circuit(qc)
+    gates = lc.gates()
+    assert len(gates) == len(qc.gates())
+    for l in range(lc.n_layers()):
+        qs = [q for g in gates[lc.layer_start[l]:lc.layer_start[l + 1]] for q in g.qubits()]
+        assert len(qs) == len(set(qs))  # one layer: disjoint qubits
+    # unitary parts in layer order equal the circuit in its own order
+    init = np.asarray(cs.random_statevector(5, 3))
+    a, b = init.copy(), init.copy()
+    for g in qc.gates():
+        oracle.apply_gate(a, np.asarray(g.matrix()), list(g.qubits()))
+    for g in gates:
+        oracle.apply_gate(b, np.asarray(g.matrix()), list(g.qubits()))
+    assert rel_l2(a, b) < 1e-13
+
+
+def test_sampler_draws_noise_branches_at_their_probabilities():
+    qc = noisy_circuit(6, 4, 2, pxx=0.1)
+    lc = cs.LayeredCircuit.from_circuit(qc)
+    gates = lc.gates()
+    noisy = [i for i, g in enumerate(gates) if not g.is_unitary()]
+    count = 3000
+    S = cs.sample_trajectories(lc, cs.NoiseSpec(0.0, lc.n_layers(), lc.n_layers(), 5), count)
+    hits = {}
+    for s in S:
+        for g, b, kind in s.errors:
+            assert kind == 0 and g in noisy
+            hits[(g, b)] = hits.get((g, b), 0) + 1
+    for g in noisy:
+        clean = cs.clean_branch(gates[g])
+        probs = [p for p, _ in gates[g].noise_branches()]
+        assert clean == int(np.argmax(probs))
+        for b, p in enumerate(probs):
+            if b == clean:
+                assert (g, b) not in hits
+                continue
+            mu, sd = count * p, np.sqrt(count * p * (1 - p))
+            assert abs(hits.get((g, b), 0) - mu) < 5 * sd, (g, b)
+    # branch events and depolarizing Paulis together: branch first per gate
+    S2 = cs.sample_trajectories(lc, cs.NoiseSpec(0.2, 1, lc.n_layers(), 9), 200)
+    for s in S2:
+        for e0, e1 in zip(s.errors, s.errors[1:]):
+            assert e0[0] <= e1[0]
+            if e0[0] == e1[0]:
+                assert e1[2] != 0
+
+
+def test_trajectory_circuit_takes_the_sampled_branch():
+    qc = noisy_circuit(4, 2, 6)
+    lc = cs.LayeredCircuit.from_circuit(qc)
+    gates = lc.gates()
+    g = next(i for i, x in enumerate(gates) if x.n_qubits() == 2 and not x.is_unitary())
+    s = cs.TrajectorySample(lc.n_layers(), [(g, 1, 0), (g, gates[g].qubits()[1], 3)])
+    out = cs.trajectory_circuit(lc, s).gates()
+    k1 = np.asarray(gates[g].noise_branches()[1][1])
+    assert np.allclose(np.asarray(out[g].matrix()), k1)
+    assert np.allclose(np.asarray(out[g + 1].matrix()), PAULI[3])
+    k0 = np.asarray(gates[g].noise_branches()[0][1])
+    clean = cs.trajectory_circuit(lc, cs.TrajectorySample(lc.n_layers(), [])).gates()
+    assert np.allclose(np.asarray(clean[g].matrix()), k0)
+    with pytest.raises(cs.CastError):  # a branch event on a unitary gate
+        cs.trajectory_circuit(lc, cs.TrajectorySample(lc.n_layers(), [(0, 1, 0)]))
+
+
+@pytest.mark.parametrize("B,kmax", [(1, 2), (2, 4), (3, 3), (2, 5)])
+def test_plan_with_noise_models_matches_explicit_trajectories(B, kmax):
+    qc = noisy_circuit(6, 4, 19)
+    lc = cs.LayeredCircuit.from_circuit(qc)
+    D = lc.n_layers()
+    S = cs.sample_trajectories(lc, cs.NoiseSpec(0.05, 1, D, 300 + B), 40)
+    assert sum(1 for s in S for e in s.errors if e[2] == 0) > 20
+    init = np.asarray(cs.random_statevector(6, 4))
+    prec = cs.Precision.F32 if kmax == 5 else cs.Precision.F64
+    eng = cs.TrajectoryEngine(lc, prec, [0, 1])
+    eng.set_initial_state(list(init))
+    P = eng.plan(S, cs.TrajectoryTuning(B, kmax, 256, 2, 0))
+    pool = P["pool"]
+    clean = cs.trajectory_circuit(lc, cs.TrajectorySample(D, [])).gates()
+    for t, s in enumerate(S):
+        want = oracle_run(cs.trajectory_circuit(lc, s).gates(), init)
+        first, cnt, c = P["tasks"][P["task_of"][t]]
+        got = oracle_run(clean[:lc.layer_start[c * B]], init)
+        for off, q in P["gates"][first:first + cnt]:
+            K = 1 << len(q)
+            if K >= 16:
+                reals = pool.view(np.float64)[2 * off:2 * off + 3 * K * K]
+                m = (reals[:K * K] + 1j * reals[K * K:2 * K * K]).reshape(K, K)
+            else:
+                m = pool[off:off + K * K].reshape(K, K)
+            oracle.apply_gate(got, m, list(q))
+        assert rel_l2(want, got) < 1e-12, f"trajectory {t}"
+
+
+def test_noise_model_trajectories_converge_to_density_matrix():
+    """Non-unitary branches: the ratio of summed unnormalised expectations to
+    summed squared norms converges to the density-matrix value (each noisy
+    gate lifted to sum_b p_b K_b (x) conj(K_b), src/types/gate.rs:236-258),
+    within 3 sigma."""
+    n, count = 4, 4000
+    qc = noisy_circuit(n, 3, 44, gamma=0.5, pxx=0.2)
+    lc = cs.LayeredCircuit.from_circuit(qc)
+    D = lc.n_layers()
+    pairs = [(i, i + 1) for i in range(n - 1)]
+    idx = np.arange(1 << n)
+    zzv = np.mean([1 - 2 * (((idx >> a) ^ (idx >> b)) & 1) for a, b in pairs], axis=0)
+    v = np.zeros(2 << (2 * n))
+    v[0] = 1.0
+    for g in lc.gates():
+        sg = g.to_density_matrix_gate(n)
+        oracle.apply_gate(v, np.asarray(sg.matrix()), list(sg.qubits()))
+    rho = to_complex(v).reshape(1 << n, 1 << n).T
+    assert abs(np.trace(rho) - 1) < 1e-12
+    exact = float(np.real(np.sum(np.diag(rho) * zzv)))
+    S = cs.sample_trajectories(lc, cs.NoiseSpec(0.0, D, D, 77), count)
+    z0 = np.zeros(2 << n)
+    z0[0] = 1.0
+    zz, nrm = [], []
+    for s in S:
+        pr = np.abs(to_complex(oracle_run(cs.trajectory_circuit(lc, s).gates(), z0))) ** 2
+        zz.append(float(np.sum(pr * zzv)))
+        nrm.append(float(pr.sum()))
+    zz, nrm = np.array(zz), np.array(nrm)
+    assert nrm.std() > 0.05  # the branches really change the norm
+    ratio = zz.sum() / nrm.sum()
+    sigma = np.std(zz - ratio * nrm, ddof=1) / np.sqrt(count) / nrm.mean()
+    assert abs(ratio - exact) < 3 * sigma + 1e-12, (ratio, exact, sigma)
