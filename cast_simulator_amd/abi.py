@@ -131,6 +131,8 @@ _SIGS = {
     "cast_hip_observe_scratch_bytes": (_sz, [_u32]),
     "cast_hip_observe_async": (
         _i, [_u64, _u32, _u8, C.POINTER(_u32), _sz, _u64, _u64, _vp, _err, _sz]),
+    "cast_hip_observe_noalloc": (
+        _i, [_u64, _u32, _u8, C.POINTER(_u32), _sz, _u64, _vp, _err, _sz]),
     "cast_hip_stream_destroy": (None, [_vp]),
     "cast_hip_stream_sync": (_i, [_vp, _err, _sz]),
     "cast_hip_event_create": (_vp, [_err, _sz]),
@@ -391,6 +393,13 @@ def observe_async(dptr: int, n_qubits: int, precision: int, pairs, out: int, scr
     arr = (_u32 * max(1, len(flat)))(*flat)
     _call("cast_hip_observe_async", dptr, n_qubits, precision, arr, len(pairs), out, scratch,
           stream)
+
+
+def observe_noalloc(dptr: int, n_qubits: int, precision: int, pairs, out: int,
+                    stream: int | None = None) -> None:
+    flat = [q for p in pairs for q in p]
+    arr = (_u32 * max(1, len(flat)))(*flat)
+    _call("cast_hip_observe_noalloc", dptr, n_qubits, precision, arr, len(pairs), out, stream)
 
 
 def inner_product(a: int, prec_a: int, b: int, prec_b: int, n_qubits: int,

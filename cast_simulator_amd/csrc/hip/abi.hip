@@ -724,6 +724,30 @@ int cast_hip_observe_async(uint64_t sv_dptr, uint32_t sv_n_qubits, uint8_t preci
   });
 }
 
+int cast_hip_observe_noalloc(uint64_t sv_dptr, uint32_t sv_n_qubits, uint8_t precision,
+                             const uint32_t *pairs, size_t n_pairs, uint64_t out, void *stream,
+                             char *err, size_t len) {
+  return guard(err, len, [&] {
+    check_precision(precision);
+    if (!sv_dptr || !out)
+      throw Error("null statevector or output pointer");
+    check_aligned(sv_dptr);
+    if (n_pairs > observe_noalloc_max_pairs())
+      throw Error("at most " + std::to_string(observe_noalloc_max_pairs()) +
+                  " qubit pairs (size of the device scratch)");
+    if (n_pairs > 0 && !pairs)
+      throw Error("null pairs");
+    if (sv_n_qubits < 1 || sv_n_qubits > 40)
+      throw Error("statevector qubit count out of range");
+    for (size_t p = 0; p < 2 * n_pairs; ++p)
+      if (pairs[p] >= sv_n_qubits)
+        throw Error("pair qubit " + std::to_string(pairs[p]) + " outside the statevector");
+    launch_observe_noalloc(reinterpret_cast<const void *>(sv_dptr), sv_n_qubits, precision, pairs,
+                           n_pairs, reinterpret_cast<double *>(out),
+                           static_cast<hipStream_t>(stream));
+  });
+}
+
 int cast_hip_inner_product(uint64_t a, uint8_t precision_a, uint64_t b, uint8_t precision_b,
                            uint32_t sv_n_qubits, double *out_re_im, void *stream, char *err,
                            size_t len) {

@@ -137,6 +137,12 @@ void dm_populations(const void *sv, uint32_t n_physical, uint8_t precision, doub
 size_t observe_scratch_bytes(uint32_t n_pairs);
 void launch_observe(const void *sv, uint32_t n_qubits, uint8_t precision, const uint32_t *pairs,
                     size_t n_pairs, double *out, double *scratch, hipStream_t stream);
+// The same with the library's static device scratch (no allocation; at most
+// observe_noalloc_max_pairs() pairs).
+size_t observe_noalloc_max_pairs();
+void launch_observe_noalloc(const void *sv, uint32_t n_qubits, uint8_t precision,
+                            const uint32_t *pairs, size_t n_pairs, double *out,
+                            hipStream_t stream);
 
 // Device blocks for small long-lived objects (gate matrices), cached by
 // power-of-two size class: a released block is kept for reuse rather than

@@ -10,6 +10,7 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -347,7 +348,7 @@ struct RedScratch {
           "hipHostMalloc(out)");
   }
 };
-RedScratch &scratch() {
+RedScratch &red_scratch() {
   static RedScratch s;
   return s;
 }
@@ -366,7 +367,7 @@ void launch_sv_zero(void *sv, size_t n_elements, uint8_t precision, hipStream_t 
 }
 
 double norm_squared(const void *sv, size_t n_elements, uint8_t precision, hipStream_t stream) {
-  RedScratch &s = scratch();
+  RedScratch &s = red_scratch();
   std::lock_guard<std::mutex> lock(s.mu);
   s.ensure();
   const unsigned grid = red_grid(n_elements / (precision == 0 ? 4 : 2));
@@ -405,7 +406,7 @@ void launch_scale(void *sv, size_t n_elements, uint8_t precision, double scale,
 
 void expect_zz(const void *sv, uint32_t n_qubits, uint8_t precision, const uint32_t *pairs,
                size_t n_pairs, double *out, hipStream_t stream) {
-  RedScratch &s = scratch();
+  RedScratch &s = red_scratch();
   std::lock_guard<std::mutex> lock(s.mu);
   s.ensure();
   const uint64_t n_amp = uint64_t(1) << n_qubits;
@@ -438,6 +439,57 @@ void expect_zz(const void *sv, uint32_t n_qubits, uint8_t precision, const uint3
 
 size_t observe_scratch_bytes(uint32_t n_pairs) {
   return sizeof(double) * kMaxRedGrid * (1 + std::min<uint32_t>(n_pairs, kMaxPairs));
+}
+
+namespace {
+// Reduction partials of the allocation-free observe call: room for the
+// largest launch_observe (kMaxRedGrid blocks x (1 + kMaxPairs) slots, 528
+// KiB).  Each GPU the library runs on holds its own copy until the process
+// ends.  Every slot a call's sum kernel reads is first written by the same
+// call's k_observe, so nothing needs clearing; calls that overlap on one GPU
+// share it and race.
+static __device__ __align__(256) double scratch[size_t(kMaxRedGrid) * (1 + kMaxPairs)];
+
+constexpr int kMaxDevices = 64;
+std::atomic<double *> g_scratch_ptr[kMaxDevices];
+std::mutex g_scratch_mu;
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current (the host-side name is not a device
+// address and must never be passed to a kernel).
+double *device_scratch() {
+  int dev = 0;
+  check(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices)
+    throw Error("device ordinal " + std::to_string(dev) + " out of range");
+  double *p = g_scratch_ptr[dev].load(std::memory_order_acquire);
+  if (p)
+    return p;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  p = g_scratch_ptr[dev].load(std::memory_order_relaxed);
+  if (!p) {
+    void *addr = nullptr;
+    check(hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!addr)
+      throw Error("hipGetSymbolAddress(scratch) returned a null address");
+    p = static_cast<double *>(addr);
+    g_scratch_ptr[dev].store(p, std::memory_order_release);
+  }
+  return p;
+}
+} // namespace
+
+size_t observe_noalloc_max_pairs() { return kMaxPairs; }
+
+void launch_observe_noalloc(const void *sv, uint32_t n_qubits, uint8_t precision,
+                            const uint32_t *pairs, size_t n_pairs, double *out,
+                            hipStream_t stream) {
+  if (n_pairs > kMaxPairs)
+    throw Error("observe (no allocation): at most " + std::to_string(kMaxPairs) +
+                " qubit pairs, the size of the device scratch; got " + std::to_string(n_pairs));
+  static_assert(sizeof(scratch) >= sizeof(double) * kMaxRedGrid * (1 + kMaxPairs),
+                "scratch must hold the largest launch_observe");
+  launch_observe(sv, n_qubits, precision, pairs, n_pairs, out, device_scratch(), stream);
 }
 
 void launch_observe(const void *sv, uint32_t n_qubits, uint8_t precision, const uint32_t *pairs,
@@ -491,7 +543,7 @@ void launch_inner(const void *a, uint8_t prec_a, const void *b, uint8_t prec_b, 
 
 void inner_product(const void *a, uint8_t prec_a, const void *b, uint8_t prec_b,
                    uint32_t n_qubits, double *out, hipStream_t stream) {
-  RedScratch &s = scratch();
+  RedScratch &s = red_scratch();
   std::lock_guard<std::mutex> lock(s.mu);
   s.ensure();
   launch_inner(a, prec_a, b, prec_b, n_qubits, s.d_out, s.d_partial, stream);

@@ -260,6 +260,20 @@ int cast_hip_observe_async(uint64_t sv_dptr, uint32_t sv_n_qubits, uint8_t preci
                            const uint32_t *pairs, size_t n_pairs, uint64_t out, uint64_t scratch,
                            void *stream, char *err_buf, size_t err_buf_len);
 
+/* cast_hip_observe_async without the scratch argument: the reduction
+ * partials live in a static device array of the library (528 KiB per GPU,
+ * held until the process ends), so the call allocates nothing (no hipMalloc,
+ * no pinned memory) and only enqueues two kernels on `stream`.  The caller
+ * passes the statevector and the device output (1 + n_pairs doubles);
+ * n_pairs above 32 is rejected.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (other streams, other threads) race
+ * and must be serialised by the caller.  No counterpart in the reference:
+ * it is for callers that cannot allocate (code under stream capture, tight
+ * serving loops, harnesses that forbid hipMalloc). */
+int cast_hip_observe_noalloc(uint64_t sv_dptr, uint32_t sv_n_qubits, uint8_t precision,
+                             const uint32_t *pairs, size_t n_pairs, uint64_t out, void *stream,
+                             char *err_buf, size_t err_buf_len);
+
 /* Marginal measurement probabilities over `measured_qubits` (distinct, any
  * order): out[b] = sum of |amp_j|^2 over every j whose bit measured_qubits[i]
  * equals bit i of b, for b < 2^n_measured, accumulated in f64 on the device;
