@@ -18,6 +18,8 @@ import sys
 from pathlib import Path
 
 DEFAULT_DIR = Path(__file__).resolve().parent / "profiles"
+# R^2 below which the fit is reported as unreliable (src/profile.rs:69, 323-328).
+R2_WARN = 0.90
 
 
 def auto_qubits(precision: str) -> int:
@@ -31,6 +33,15 @@ def auto_qubits(precision: str) -> int:
     while n > 10 and (2 << n) * scalar > 0.8 * free:
         n -= 1
     return n
+
+
+def r2_warning(label: str, r2: float):
+    """The reference's warning for an unreliable fit (src/profile.rs:323-328),
+    or None when R^2 >= R2_WARN."""
+    if r2 >= R2_WARN:
+        return None
+    return (f"  {label}: Warning: roofline fit R^2 = {r2:.3f} (< {R2_WARN}); "
+            "crossover estimate may be unreliable.")
 
 
 def profile(precision: str, n: int, budget: float, seed: int = 1):
@@ -61,6 +72,9 @@ def main(argv=None) -> int:
         print(f"  crossover AI {prof.crossover_ai:.2f}, bandwidth slope "
               f"{prof.bw_slope:.2f} GFLOP/s per AI, peak {prof.peak_gflops_s:.1f} GFLOP/s "
               f"(reference units), {len(prof.raw)} probes, R^2 {r2:.3f} -> {out}", flush=True)
+        warn = r2_warning(f"hip {p}", r2)
+        if warn:
+            print(warn, flush=True)
     return 0
 
 

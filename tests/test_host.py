@@ -301,3 +301,21 @@ def test_fusion_log_and_adaptive_rejects_past_crossover():
     g2 = cs.CircuitGraph.from_circuit(c)
     cs.optimize(g2, cs.FusionConfig.size_only(5))
     assert len(g2.gates_in_row_order()) <= len(g.gates_in_row_order())
+
+
+def test_r2_warning_below_threshold():
+    """profile_hw reports an unreliable fit below R^2 0.90 with the
+    reference's wording (src/profile.rs:69, 323-328): a sweep with a dip
+    between the memory slope and the plateau fits worse than that, the ideal
+    roofline does not."""
+    from cast_simulator_amd import profile_hw
+
+    ideal = [(a, min(500 * a, 20000), 0) for a in (1, 2, 4, 8, 16, 32, 64, 128)]
+    dip = [(a, min(500 * a, 20000) * (0.45 if 24 <= a <= 64 else 1.0), 0)
+           for a in (1, 2, 4, 8, 16, 24, 32, 48, 64, 96, 128)]
+    r2_ideal, r2_dip = cs.fit_roofline(ideal)[2], cs.fit_roofline(dip)[2]
+    assert r2_dip < profile_hw.R2_WARN <= r2_ideal
+    assert profile_hw.r2_warning("hip f32", r2_ideal) is None
+    msg = profile_hw.r2_warning("hip f32", r2_dip)
+    assert msg == (f"  hip f32: Warning: roofline fit R^2 = {r2_dip:.3f} (< 0.9); "
+                   "crossover estimate may be unreliable.")
