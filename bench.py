@@ -34,7 +34,18 @@ HBM_PEAK_GBS = 8000.0
 SEED = 20260101
 KERNEL = {1: "k_dense", 2: "k_monomial", 3: "k_tile", 4: "k_mfma", 5: "k_tile(swizzled)",
           6: "k_tile(3 products)", 7: "k_tile(3 products, swizzled)", 8: "k_tile(xcd order)",
-          9: "k_tile(3 products, xcd order)"}
+          9: "k_tile(3 products, xcd order)", 10: "k_mfma(f32 products)"}
+
+
+def mfma_symbol(f: int, ctype: str, k: int) -> str:
+    """Profiler symbol of a matrix-core or monomial launch: f32 k = 5, 6 run
+    the bf16 split-product kernel unless the f32-product kernel (code 10) was
+    chosen."""
+    if f == 4 and ctype == "float" and k >= 5:
+        return f"k_mfma_split<{k}>"
+    if f in (4, 10):
+        return f"k_mfma6<{ctype}>" if k == 6 else f"k_mfma<{ctype}, {k}>"
+    return f"{KERNEL[f]}<{ctype}, {k}>"
 
 
 def log(msg: str) -> None:
@@ -99,7 +110,7 @@ def run_gpu(cs, n: int, precision: str, steps: int, warmup: int, max_size: int):
         if f == 1:  # dense: the store mode is a template argument
             nts = precision == "f64" and min(info[kid].qubits) >= 3
             return f"k_dense<{ctype}, {k}, {'true' if nts else 'false'}>"
-        return f"{KERNEL[f]}<{ctype}, {k}>"
+        return mfma_symbol(f, ctype, k)
 
     per_kernel = defaultdict(list)
     t0 = time.perf_counter()
@@ -286,14 +297,17 @@ def run_dense_large(cs, n64: int = 33, n32: int = 34, depth: int = 12):
         sb = 8 if prec == "f64" else 4
         B = 2 * (1 << n) * 2 * sb
         groups = defaultdict(list)
+        ctype = "double" if prec == "f64" else "float"
         for kid, t in kernels:
             k = info[kid].k
             dense = info[kid].scalar_nnz == 2 * (1 << k) ** 2
-            groups[(KERNEL[fam[kid]], k, dense)].append(t)
+            f = fam[kid]
+            kern = mfma_symbol(f, ctype, k) if f in (4, 10) else f"{KERNEL[f]}<{prec}, k={k}>"
+            groups[(kern, k, dense)].append(t)
         rows = []
         for (kern, k, dense), ts in sorted(groups.items(), key=lambda kv: -sum(kv[1])):
             avg = sum(ts) / len(ts)
-            row = {"kernel": f"{kern}<{prec}, k={k}>" + ("" if dense else " (sparse)"),
+            row = {"kernel": kern + ("" if dense else " (sparse)"), "k": k,
                    "launches": len(ts), "avg_ms": round(avg * 1e3, 3),
                    "gbs": round(B / avg / 1e9, 1)}
             if dense:
@@ -322,10 +336,20 @@ def run_dense_large(cs, n64: int = 33, n32: int = 34, depth: int = 12):
     log(f"  {r32['fused_gates']} gates in {r32['wall_s']} s, {r32['gbs_per_gate']} GB/s per gate")
 
     def roof(r, prec):
-        k6 = [x for x in r["kernels"] if "k=6>" in x["kernel"] and "tflops" in x]
+        k6 = [x for x in r["kernels"] if x["k"] == 6 and "tflops" in x]
         if not k6:
             return None
         top = max(k6, key=lambda x: x["launches"] * x["avg_ms"])
+        if "split" in top["kernel"]:
+            # bf16 split products: 4 real products x 6 partial products on
+            # the bf16 matrix cores = 6x the algorithmic flops; the kernel is
+            # HBM-bound (bf16 peak 2.5 PFLOP/s dense)
+            return {"bound": "hbm", "kernel": top["kernel"], "achieved": top["gbs"],
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(top["gbs"] / HBM_PEAK_GBS, 4),
+                    "algorithmic_tflops": top["tflops"],
+                    "bf16_mfma_tflops": round(6 * top["tflops"], 1),
+                    "bf16_mfma_frac": round(6 * top["tflops"] / 2500.0, 4), "traffic": None}
         peak = 78.6 if prec == "f64" else 157.3
         return {"bound": "mfma", "kernel": top["kernel"], "achieved": top["tflops"],
                 "peak": peak, "unit": "TFLOP/s", "frac": round(top["tflops"] / peak, 4),
@@ -512,7 +536,8 @@ def run_trajectories(cs, n: int, count: int, tune_sample: int, plain_sample: int
         "slots": r.n_slots,
         "tuning": {"chunk_layers": best.chunk_layers, "fusion_max": best.fusion_max,
                    "block": best.block, "slots_per_cu": best.slots_per_cu,
-                   "waves_per_simd": best.waves_per_simd},
+                   "waves_per_simd": best.waves_per_simd,
+                   "k5_k6_products": "f32" if best.mfma_f32 else "bf16 split"},
         "autotune": {"trials": len(trials), "sample": tune_sample, "seconds": round(t_tune, 1)},
         "kernel_gbs": round(alg / r.gpu_s / 1e9, 1),
         "kernel_hbm_frac": round(alg / r.gpu_s / 1e9 / HBM_PEAK_GBS, 4),
@@ -558,10 +583,12 @@ def run_serving(cs, count: int, rate: float, capacity_sample: int):
     reqs = cs.poisson_requests(count, rate, 12, 28, 10, 50, 0.5, SEED)
     streams, threads = 4, 16
 
-    def serve(rs, n_streams, realtime, shadow=False):
+    def serve(rs, n_streams, realtime, shadow=False, batch=True):
         cfg = cs.ServeConfig()
         cfg.streams, cfg.compile_threads, cfg.realtime = n_streams, threads, realtime
         cfg.f64_shadow = shadow
+        if not batch:  # every request on the per-request path
+            cfg.batch_max_qubits = 0
         return cs.Server(cfg).run(rs)
 
     res, st = serve(reqs, streams, True)
@@ -569,17 +596,21 @@ def run_serving(cs, count: int, rate: float, capacity_sample: int):
         f"p50 {st['p50_s'] * 1e3:.1f} ms, p99 {st['p99_s'] * 1e3:.1f} ms")
     sub = reqs[:capacity_sample]
     _, cap = serve(sub, streams, False)
+    _, cap_nb = serve(sub, streams, False, batch=False)
     # Plain per-circuit path: the same circuits in f64, one at a time on one
     # stream, compiled ahead by the same host threads.
-    _, pl = serve([r.with_precision(cs.Precision.F64) for r in sub], 1, False)
-    log(f"  capacity: {cap['throughput']:.1f} circuits/s mixed, plain f64 serial "
-        f"{pl['throughput']:.1f} circuits/s")
+    _, pl = serve([r.with_precision(cs.Precision.F64) for r in sub], 1, False, batch=False)
+    log(f"  capacity: {cap['throughput']:.1f} circuits/s mixed ({cap_nb['throughput']:.1f} "
+        f"without the batch lane), plain f64 serial {pl['throughput']:.1f} circuits/s")
     # Latency below saturation: a second stream at 70 % of the measured
     # capacity (the offered load at `rate` exceeds what the HBM can serve).
     sus_rate = round(0.7 * cap["throughput"], 1)
     sus_reqs = cs.poisson_requests(min(300, count), sus_rate, 12, 28, 10, 50, 0.5, SEED + 1)
     _, sus = serve(sus_reqs, streams, True)
-    log(f"  at {sus_rate}/s: p50 {sus['p50_s'] * 1e3:.1f} ms, p99 {sus['p99_s'] * 1e3:.1f} ms")
+    _, sus_nb = serve(sus_reqs, streams, True, batch=False)
+    log(f"  at {sus_rate}/s: p50 {sus['p50_s'] * 1e3:.1f} ms, p99 {sus['p99_s'] * 1e3:.1f} ms "
+        f"(without the batch lane: p50 {sus_nb['p50_s'] * 1e3:.1f} ms, "
+        f"p99 {sus_nb['p99_s'] * 1e3:.1f} ms)")
     # Accuracy pass: every request again, each f32 one with its f64 replay on
     # the device (checking mode, not timed).
     shadow, _ = serve(reqs, streams, False, shadow=True)
@@ -591,7 +622,9 @@ def run_serving(cs, count: int, rate: float, capacity_sample: int):
     return {
         "workload": f"{count} requests, Poisson arrivals at {rate}/s (seed {SEED}), "
                     "random-layer circuits n ~ U{12..28}, depth ~ U{10..50}, f32/f64 50/50, "
-                    "fused to k<=6, resident statevectors, admission by free HBM, 4 streams",
+                    "fused to k<=6, resident statevectors, admission by free HBM, 4 streams; "
+                    "requests of <= 18 qubits in batches of up to 32 (one launch per gate "
+                    "step and size)",
         "circuits_per_s": round(st["throughput"], 2),
         "p50_ms": round(st["p50_s"] * 1e3, 2),
         "p99_ms": round(st["p99_s"] * 1e3, 2),
@@ -601,7 +634,10 @@ def run_serving(cs, count: int, rate: float, capacity_sample: int):
         "peak_sv_gib": round(st["peak_sv_bytes"] / 2**30, 2),
         "wall_s": round(st["wall_s"], 2),
         "capacity_circuits_per_s": round(cap["throughput"], 2),
+        "capacity_without_batch_lane": round(cap_nb["throughput"], 2),
         "capacity_sample": len(sub),
+        "batch_lane": {"batches": st["batches"], "batched_requests": st["batched_requests"],
+                       "launches": st["batch_launches"]},
         "plain_f64_serial_circuits_per_s": round(pl["throughput"], 2),
         "speedup_vs_plain_f64": round(cap["throughput"] / pl["throughput"], 2),
         "max_norm_error": max(norms),
@@ -615,7 +651,11 @@ def run_serving(cs, count: int, rate: float, capacity_sample: int):
                              "circuits_per_s": round(sus["throughput"], 2),
                              "p50_ms": round(sus["p50_s"] * 1e3, 2),
                              "p99_ms": round(sus["p99_s"] * 1e3, 2),
-                             "max_in_flight": sus["max_in_flight"]},
+                             "max_in_flight": sus["max_in_flight"],
+                             "without_batch_lane": {
+                                 "circuits_per_s": round(sus_nb["throughput"], 2),
+                                 "p50_ms": round(sus_nb["p50_s"] * 1e3, 2),
+                                 "p99_ms": round(sus_nb["p99_s"] * 1e3, 2)}},
     }
 
 
@@ -732,7 +772,7 @@ def main() -> int:
                          "trajectories: the batched-trajectory launch alone with a fixed "
                          "tuning (for profiling); each prints its own JSON line")
     ap.add_argument("--traj-tuning", default="4,6,512,1,2",
-                    help="chunk_layers,fusion_max,block,slots_per_cu,waves_per_simd for "
+                    help="chunk_layers,fusion_max,block,slots_per_cu,waves_per_simd[,mfma_f32] for "
                          "--workload trajectories")
     ap.add_argument("--no-traj", action="store_true")
     ap.add_argument("--no-large", action="store_true",
@@ -772,9 +812,11 @@ def main() -> int:
         print(json.dumps(dense_large_line(run_dense_large(cs))), flush=True)
         return 0
     if args.workload == "trajectories":
-        b, k, blk, spc, occ = (int(x) for x in args.traj_tuning.split(","))
+        vals = [int(x) for x in args.traj_tuning.split(",")]
+        b, k, blk, spc, occ = vals[:5]
+        mf32 = vals[5] if len(vals) > 5 else 0
         print(json.dumps(run_trajectory_launch(cs, args.traj_qubits, args.trajectories,
-                                               cs.TrajectoryTuning(b, k, blk, spc, 0, occ))),
+                                               cs.TrajectoryTuning(b, k, blk, spc, 0, occ, mf32))),
               flush=True)
         return 0
 
@@ -876,9 +918,20 @@ def main() -> int:
     if fid:
         line["fidelity_f32_vs_f64"] = fid["fidelity_vs_f64"]
     if r32:
+        d32 = r32["dominant"]
+        t32 = None
+        tpath32 = ROOT / "profiles" / "gate_f32_30q_traffic.json"
+        if args.qubits == 30 and tpath32.exists():
+            t32 = json.loads(tpath32.read_text())["hbm_bytes_per_launch"].get(d32["kernel"])
         line["f32"] = {
             "gbs_per_gate": round(r32["gbs_per_gate"], 2),
             "ms_per_step": round(r32["ms_per_step"], 3),
+            "roofline": {"bound": "hbm", "kernel": d32["kernel"],
+                         "achieved": round(d32["achieved_gbs"], 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(d32["achieved_gbs"] / HBM_PEAK_GBS, 4),
+                         "traffic": t32,
+                         "traffic_source": "profiles/gate_f32_30q_traffic.json" if t32 else None,
+                         "avg_launch_ms": round(d32["avg_ms"], 4), "launches": d32["launches"]},
             "per_kernel_gbs": {k: round(v, 1) for k, v in r32["per_k_gbs"].items()},
             "fused_gates": r32["n_gates"],
             "k_histogram": r32["k_hist"],

@@ -42,6 +42,8 @@ class KernelGenSpec(C.Structure):
 FLAG_TILE_SWIZZLE = 1
 FLAG_TILE_GAUSS = 2
 FLAG_TILE_XCD = 4
+FLAG_MFMA_SPLIT = 8
+FLAG_MFMA_F32 = 16
 
 
 class PlanInfo(C.Structure):
@@ -111,7 +113,7 @@ class TrajLaunch(C.Structure):
         ("dump", C.c_uint64),
         ("work", C.c_uint64),
         ("waves_per_simd", C.c_uint32),
-        ("reserved", C.c_uint32),
+        ("flags", C.c_uint32),
     ]
 
 

@@ -174,7 +174,10 @@ GatePlan *build_plan(const cast_hip_kernel_gen_spec_t *spec, const cast_complex6
     variant = CAST_HIP_VARIANT_DENSE; // matrix-core tiles exist for k = 4, 5, 6
   plan->variant = variant;
   plan->forced = spec->variant != CAST_HIP_VARIANT_AUTO;
-  if (spec->flags & ~(CAST_HIP_FLAG_TILE_SWIZZLE | CAST_HIP_FLAG_TILE_GAUSS | CAST_HIP_FLAG_TILE_XCD))
+  if ((spec->flags & CAST_HIP_FLAG_MFMA_SPLIT) && (spec->flags & CAST_HIP_FLAG_MFMA_F32))
+    throw Error("flags MFMA_SPLIT and MFMA_F32 exclude each other");
+  if (spec->flags & ~(CAST_HIP_FLAG_TILE_SWIZZLE | CAST_HIP_FLAG_TILE_GAUSS | CAST_HIP_FLAG_TILE_XCD |
+                      CAST_HIP_FLAG_MFMA_SPLIT | CAST_HIP_FLAG_MFMA_F32))
     throw Error("unknown flags " + std::to_string(spec->flags));
   plan->flags = spec->flags;
 

@@ -339,5 +339,170 @@ __device__ __forceinline__ Acc4<double>::type mfma16(double a, double b, Acc4<do
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
+// ── bf16 split products (f32 dense k = 5, 6 on the bf16 matrix cores) ─────
+//
+// Every f32 operand x is split into three bf16 parts x = x0 + x1 + x2 + r,
+// |r| <= 2^-27 |x| (each part the round-to-nearest bf16 of what the earlier
+// parts leave; those differences are exact in f32).  A product a b is the
+// sum of the six partial products a_s b_t with s + t <= 2 (the dropped ones
+// are below 2^-24 |a b|) on v_mfma_f32_16x16x32_bf16 with f32 accumulation,
+// smallest first.  The complex product is four real products (Re M vr,
+// Im M vi, and Re M vi + Im M vr into one accumulator), so only the splits
+// of Re M and Im M are staged in LDS (k = 5: 12 KiB, k = 6: 48 KiB).
+//
+// v_mfma_f32_16x16x32_bf16: lane l supplies A[l % 16][8 (l / 16) + e] and
+// B[8 (l / 16) + e][l % 16] as element e of an 8-vector and receives, in
+// accumulator register i, D[4 (l / 16) + i][l % 16].  Lane l owns combo
+// l % 16 of a 16-combo tile, loads input rows 32 ks + 8 (l / 16) + e and
+// stores output rows 16 rb + 4 (l / 16) + i.
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f32x4_t mfma_bf16(bf16x8_t a, bf16x8_t b, f32x4_t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+struct Split3 {
+  __bf16 p[3];
+};
+
+__device__ __forceinline__ Split3 split3(float x) {
+  Split3 s;
+  s.p[0] = __bf16(x);
+  const float r1 = x - float(s.p[0]);
+  s.p[1] = __bf16(r1);
+  const float r2 = r1 - float(s.p[1]);
+  s.p[2] = __bf16(r2);
+  return s;
+}
+
+template <int KL> struct SplitOperands {
+  static_assert(KL == 5 || KL == 6, "bf16 split products for k = 5, 6");
+  static constexpr int K = 1 << KL, RB = K / 16, KS = K / 32;
+  // fragment (matrix m: 0 = Re, 1 = Im; split s; row block rb; k step ks)
+  // of lane l at index (((m * 3 + s) * RB + rb) * KS + ks) * 64 + l: every
+  // read is 64 consecutive 16-byte vectors
+  static constexpr int kFragments = 2 * 3 * RB * KS * 64;
+  __device__ static constexpr int at(int m, int s, int rb, int ks) {
+    return (((m * 3 + s) * RB + rb) * KS + ks) * 64;
+  }
+};
+
+// Stage the splits of Re M and Im M (from the three real matrices
+// [Re | Im | Re + Im], K x K row-major each) into a_lds; every thread of the
+// workgroup takes part, and the caller synchronises before use.
+template <int KL>
+__device__ __forceinline__ void split_stage(const float *__restrict__ m3, bf16x8_t *a_lds) {
+  using SO = SplitOperands<KL>;
+  constexpr int K = SO::K, RB = SO::RB, KS = SO::KS;
+  for (unsigned i = threadIdx.x; i < 2 * RB * KS * 64; i += blockDim.x) {
+    const unsigned l = i & 63u, ks = (i >> 6) % KS, rb = (i >> 6) / KS % RB,
+                   m = (i >> 6) / (KS * RB);
+    const float *row = m3 + m * K * K + (16 * rb + (l & 15u)) * K + 32 * ks + 8 * (l >> 4);
+    bf16x8_t p0, p1, p2;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const Split3 sp = split3(row[e]);
+      p0[e] = sp.p[0];
+      p1[e] = sp.p[1];
+      p2[e] = sp.p[2];
+    }
+    a_lds[SO::at(m, 0, rb, ks) + l] = p0;
+    a_lds[SO::at(m, 1, rb, ks) + l] = p1;
+    a_lds[SO::at(m, 2, rb, ks) + l] = p2;
+  }
+}
+
+// Per-lane address parts of a split-product tile: input row 32 ks + 8 j + e
+// has bits 0-2 (e) -> q0..q2, 3-4 (j) -> q3, q4, 5 (ks) -> q5; output row
+// 16 rb + 4 j + i has bits 0-1 (i) -> q0, q1, 2-3 (j) -> q2, q3,
+// 4-5 (rb) -> q4, q5 (j = lane / 16).
+template <int KL> struct SplitLane {
+  uint32_t lane, col;
+  uint64_t din_j, dout_j;
+  __device__ explicit SplitLane(const uint32_t *q) {
+    const uint64_t one = 1;
+    lane = threadIdx.x & 63u;
+    col = lane & 15u;
+    const uint32_t j = lane >> 4;
+    din_j = ((j & 1u) ? one << q[3] : 0) | ((j & 2u) ? one << q[4] : 0);
+    dout_j = ((j & 1u) ? one << q[2] : 0) | ((j & 2u) ? one << q[3] : 0);
+  }
+};
+
+// U consecutive 16-combo tiles t0 .. t0 + U - 1: reads `in`, writes `out`
+// (equal for in place).  All loads of the U tiles are issued before any
+// product, so one wave keeps U times the loads in flight (for callers with
+// few waves per CU).
+template <int KL, int U = 1>
+__device__ __forceinline__ void split_tile(const float2 *in, float2 *out, const bf16x8_t *a_lds,
+                                           const uint32_t *q, const SplitLane<KL> &sl,
+                                           uint64_t t0) {
+  using SO = SplitOperands<KL>;
+  constexpr int RB = SO::RB, KS = SO::KS;
+  const uint64_t one = 1;
+  uint64_t base[U];
+  float2 x[U][KS][8];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    base[u] = combo_base<KL>((t0 + u) * 16 + sl.col, q);
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const uint64_t d = sl.din_j | local_delta<3>(uint32_t(e), q) | (ks ? one << q[5] : 0);
+        x[u][ks][e] = in[base[u] + d];
+      }
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    bf16x8_t br[3][KS], bi[3][KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const Split3 sr = split3(x[u][ks][e].x), si = split3(x[u][ks][e].y);
+#pragma unroll
+        for (int sp = 0; sp < 3; ++sp) {
+          br[sp][ks][e] = sr.p[sp];
+          bi[sp][ks][e] = si.p[sp];
+        }
+      }
+#pragma unroll
+    for (int rp = 0; rp < RB; rp += 2) {
+      f32x4_t t1[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}}, t2[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}},
+              t3[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+      // Re-read per tile (not hoisted out of the caller's tile loop into VGPRs).
+      const bf16x8_t *a = a_lds + sl.lane + opaque_u32(0);
+#pragma unroll
+      for (int sa = 2; sa >= 0; --sa)
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const bf16x8_t ar = a[SO::at(0, sa, rp + h, ks)];
+            const bf16x8_t ai = a[SO::at(1, sa, rp + h, ks)];
+#pragma unroll
+            for (int sb = 2 - sa; sb >= 0; --sb) {
+              t1[h] = mfma_bf16(ar, br[sb][ks], t1[h]);
+              t3[h] = mfma_bf16(ar, bi[sb][ks], t3[h]);
+              t2[h] = mfma_bf16(ai, bi[sb][ks], t2[h]);
+              t3[h] = mfma_bf16(ai, br[sb][ks], t3[h]);
+            }
+          }
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const uint64_t d = sl.dout_j | local_delta<2>(uint32_t(i), q) |
+                             (((rp + h) & 1) ? one << q[4] : 0) |
+                             (((rp + h) & 2) ? one << q[5] : 0);
+          out[base[u] + d] = make_float2(t1[h][i] - t2[h][i], t3[h][i]);
+        }
+    }
+  }
+}
+
 } // namespace dev
 } // namespace cast_hip

@@ -27,6 +27,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <string>
 
 namespace cast_hip {
 namespace {
@@ -326,6 +327,32 @@ __global__ void __launch_bounds__(kMfma6Block) k_mfma6(T *__restrict__ sv, const
   }
 }
 
+// ── Matrix cores, f32 k = 5, 6 through bf16 split products ────────────────
+//
+// On the f32 matrix cores (157 TFLOP/s) a dense 6-qubit f32 gate is
+// compute-bound: AI 32 flop/B against a ridge of 19.7.  The bf16 matrix
+// cores run 16x the flops; each f32 operand is split into three bf16 parts
+// and each real product takes six partial products (dev::split_tile), which
+// keeps f32 accuracy.  One wave per 16-combo tile in a grid-stride loop;
+// the splits of Re M and Im M are staged in LDS once per workgroup.
+
+constexpr unsigned kSplitBlock = 256;
+
+template <int KL>
+__global__ void __launch_bounds__(kSplitBlock) k_mfma_split(float *__restrict__ sv,
+                                                          const float *__restrict__ m3,
+                                                          GateGeom g, uint64_t n_tiles) {
+  __shared__ dev::bf16x8_t a_lds[dev::SplitOperands<KL>::kFragments];
+  dev::split_stage<KL>(m3, a_lds);
+  __syncthreads();
+  float2 *__restrict__ amp = reinterpret_cast<float2 *>(sv);
+  const dev::SplitLane<KL> sl(g.q);
+  const uint64_t n_waves = (uint64_t(gridDim.x) * blockDim.x) >> 6;
+  for (uint64_t t = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6; t < n_tiles;
+       t += n_waves)
+    dev::split_tile<KL>(amp, amp, a_lds, g.q, sl, t);
+}
+
 // ── Batched dense: item = blockIdx.z ────────────────────────────────────────
 
 constexpr int kMaxBatch = 64;
@@ -415,8 +442,16 @@ void launch_mono_t(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &g
   check(hipGetLastError(), "hipLaunchKernelGGL(k_monomial)");
 }
 
+bool use_split(const GatePlan &p);
+template <int KL>
+void launch_mfma_split(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &g);
+
 template <typename T, int KL>
 void launch_mfma_t(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &g) {
+  if constexpr (sizeof(T) == 4 && KL == 5) {
+    if (use_split(p))
+      return launch_mfma_split<5>(p, s, sv, g);
+  }
   // Grid: the three real matrices are loaded into registers once per wave,
   // and long grid-stride loops stream worse than full grids.  At f64 the
   // grid gives each wave about 16 tiles, between 8 and 128 workgroups per
@@ -444,8 +479,64 @@ void launch_mfma_t(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &g
   check(hipGetLastError(), "hipLaunchKernelGGL(k_mfma)");
 }
 
+// k = 5, 6 at f32 on the matrix cores: bf16 split products (k_mfma_split)
+// or f32 products (k_mfma<float, 5>, k_mfma6<float>).  A plan flag forces
+// one; otherwise CAST_HIP_K6_F32 / CAST_HIP_K5_F32 ("split" or "f32", for
+// measurements) or the default below.
+bool use_split(const GatePlan &p) {
+  if (p.flags & CAST_HIP_FLAG_MFMA_SPLIT)
+    return true;
+  if (p.flags & CAST_HIP_FLAG_MFMA_F32)
+    return false;
+  auto read = [](const char *name) {
+    const char *e = std::getenv(name);
+    if (e && std::string(e) == "split")
+      return 1;
+    if (e && std::string(e) == "f32")
+      return 0;
+    return -1;
+  };
+  static const int env6 = read("CAST_HIP_K6_F32"), env5 = read("CAST_HIP_K5_F32");
+  const int env = p.k == 6 ? env6 : env5;
+  // Default: the split products, which measured both more accurate (rel L2
+  // 6-9e-8 per gate against 1.6-2.1e-7) and faster on MI355X: over five
+  // target sets, k = 6 at 34 qubits 5017 GB/s against 4508 (the f32 products
+  // are matrix-core bound at 159 algorithmic TFLOP/s), k = 5 at 34 qubits
+  // 5286 against 4957, and at 30 qubits 4351 / 4180 (k = 6) and 4489 / 4405
+  // (k = 5).  High target sets can run faster with f32 products; the
+  // per-gate autotuner tries both.
+  return env >= 0 ? env == 1 : true;
+}
+
+template <int KL>
+void launch_mfma_split(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &g) {
+  const uint64_t tiles = g.combos / 16;
+  const uint64_t need = (tiles + kSplitBlock / 64 - 1) / (kSplitBlock / 64);
+  // Grid: at most 48 workgroups per CU (k = 6 keeps 48 KiB of LDS per
+  // workgroup, 3 resident per CU; each stages the split matrices once).
+  // Measured at 30 qubits on the config-3 circuit's gates (19 k = 5, 9 k = 6):
+  // caps of 3, 8, 16 and 48 per CU gave 4799 / 4766 / 4822 / 4899 GB/s at
+  // k = 5 and 4541 / 4544 / 4588 / 4633 at k = 6.
+  // CAST_HIP_SPLIT_BLOCKS_PER_CU fixes the cap for measurements.
+  static const unsigned per_cu_env = [] {
+    const char *e = std::getenv("CAST_HIP_SPLIT_BLOCKS_PER_CU");
+    long v = e && *e ? std::strtol(e, nullptr, 10) : 0;
+    return unsigned(v > 0 && v <= 1024 ? v : 0);
+  }();
+  const uint64_t n_cu = uint64_t(std::max(1, device_props().n_cu));
+  const uint64_t cap = n_cu * (per_cu_env ? per_cu_env : 48);
+  const unsigned grid = unsigned(std::max<uint64_t>(1, std::min(need, cap)));
+  hipLaunchKernelGGL((k_mfma_split<KL>), dim3(grid), dim3(kSplitBlock), 0, s,
+                     static_cast<float *>(sv), static_cast<const float *>(p.d_mat3), g, tiles);
+  check(hipGetLastError(), "hipLaunchKernelGGL(k_mfma_split)");
+}
+
 template <typename T>
 void launch_mfma6_t(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &g) {
+  if constexpr (sizeof(T) == 4) {
+    if (use_split(p))
+      return launch_mfma_split<6>(p, s, sv, g);
+  }
   const uint64_t tiles = g.combos / 16;
   const uint64_t need = (tiles + kMfma6Block / 64 - 1) / (kMfma6Block / 64);
   // 2 workgroups per CU are resident (LDS).  The grid gives each wave about

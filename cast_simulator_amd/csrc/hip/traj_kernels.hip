@@ -44,9 +44,13 @@ static_assert(sizeof(cast_hip_traj_launch_t) == 376, "cast_hip_traj_launch_t lay
 
 struct TrajScalars {
   uint64_t n_matrix_elems;
-  uint32_t n_tasks, n_gates, n_checkpoints, n_qubits, n_pairs, log2_block;
+  uint32_t n_tasks, n_gates, n_checkpoints, n_qubits, n_pairs, log2_block, flags;
   uint32_t pairs[2 * kMaxPairs];
 };
+
+// Gate operands staged in LDS by the k = 6 (f32 products) and the split-
+// product gates, one buffer shared by both: 48 KiB at f32, none at f64.
+template <typename T> constexpr size_t kGateLdsBytes = sizeof(T) == 4 ? 48 * 1024 : 16;
 
 __global__ void k_fill_u32(uint32_t *__restrict__ p, uint32_t n, uint32_t v) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -252,11 +256,10 @@ __device__ __forceinline__ void wg_mfma_gate(const float *src, float *dst,
 // overwriting operands still in use.
 __device__ __forceinline__ void wg_mfma6_gate(const float *src, float *dst,
                                               const float *__restrict__ m3, const uint32_t *q,
-                                              uint32_t n) {
+                                              uint32_t n, float *a_lds) {
   constexpr int KL = 6, K = 64, RB = 4, KS = 16;
   using VT = typename V2<float>::type;
   using AV = dev::Acc4<float>::type;
-  __shared__ float a_lds[2 * RB * KS * 64];
   for (uint32_t i = threadIdx.x; i < 2 * RB * KS * 64; i += blockDim.x) {
     const uint32_t l = i & 63u, s = (i >> 6) & (KS - 1), rb = (i >> 10) & (RB - 1), m = i >> 12;
     a_lds[i] = m3[m * K * K + (16 * rb + (l & 15u)) * K + 4 * s + (l >> 4)];
@@ -307,6 +310,29 @@ __device__ __forceinline__ void wg_mfma6_gate(const float *src, float *dst,
         }
     }
   }
+}
+
+// k = 5, 6 at f32 through bf16 split products (dev::split_tile): the splits
+// of Re M and Im M are staged in LDS once per gate (k = 6: 48 KiB, k = 5:
+// 12 KiB of the shared gate buffer); the workgroup's waves take 16-combo
+// tiles in turn.  Needs n >= KL + 4.  The caller's barrier after every gate
+// keeps the next gate's staging from overwriting operands still in use.
+template <int KL>
+__device__ __forceinline__ void wg_split_gate(const float *src, float *dst,
+                                              const float *__restrict__ m3, const uint32_t *q,
+                                              uint32_t n, dev::bf16x8_t *a_lds) {
+  // k = 5: two tiles per step, so that each lane keeps 16 loads in flight as
+  // at k = 6 (one workgroup streams the whole statevector).
+  constexpr int U = KL == 5 ? 2 : 1;
+  dev::split_stage<KL>(m3, a_lds);
+  __syncthreads();
+  const dev::SplitLane<KL> sl(q);
+  const uint32_t wave = threadIdx.x >> 6, n_waves = blockDim.x >> 6;
+  // n >= KL + 5 for k = 5 (checked by the caller), so n_tiles is a multiple of U
+  const uint64_t n_tiles = uint64_t(1) << (n - KL - 4);
+  for (uint64_t t = uint64_t(wave) * U; t < n_tiles; t += uint64_t(n_waves) * U)
+    dev::split_tile<KL, U>(reinterpret_cast<const float2 *>(src),
+                           reinterpret_cast<float2 *>(dst), a_lds, q, sl, t);
 }
 
 template <typename T, int KL>
@@ -447,6 +473,8 @@ __global__ void __launch_bounds__(kMaxBlock, OCC)
   __shared__ uint32_t s_task;
   __shared__ uint32_t s_pairs[2 * kMaxPairs];
   __shared__ double red[kMaxWaves * (1 + kMaxPairs)];
+  __shared__ __align__(16) unsigned char gate_lds[kGateLdsBytes<T>];
+  const bool f32_products = a.flags & CAST_HIP_TRAJ_FLAG_MFMA_F32;
   if (threadIdx.x < 2 * a.n_pairs)
     s_pairs[threadIdx.x] = a.pairs[threadIdx.x];
   for (;;) {
@@ -518,12 +546,20 @@ __global__ void __launch_bounds__(kMaxBlock, OCC)
         wg_gate_k<T, 4>(cur, slot, mat, q, n);
         break;
       case 5: // f32 only (checked above)
-        if constexpr (sizeof(T) == 4)
-          wg_gate_k<T, 5>(cur, slot, mat, q, n);
+        if constexpr (sizeof(T) == 4) {
+          if (f32_products || n < 11)
+            wg_gate_k<T, 5>(cur, slot, mat, q, n);
+          else
+            wg_split_gate<5>(cur, slot, mat, q, n, reinterpret_cast<dev::bf16x8_t *>(gate_lds));
+        }
         break;
       default: // k = 6, f32 only, n >= 12 (checked above)
-        if constexpr (sizeof(T) == 4)
-          wg_mfma6_gate(cur, slot, mat, q, n);
+        if constexpr (sizeof(T) == 4) {
+          if (f32_products)
+            wg_mfma6_gate(cur, slot, mat, q, n, reinterpret_cast<float *>(gate_lds));
+          else
+            wg_split_gate<6>(cur, slot, mat, q, n, reinterpret_cast<dev::bf16x8_t *>(gate_lds));
+        }
         break;
       }
       cur = slot;
@@ -609,6 +645,9 @@ void launch_traj(const cast_hip_traj_launch_t &L, hipStream_t stream) {
   a.n_qubits = L.n_qubits;
   a.n_pairs = L.n_pairs;
   a.log2_block = uint32_t(__builtin_ctz(L.block));
+  if (L.flags & ~CAST_HIP_TRAJ_FLAG_MFMA_F32)
+    throw Error("trajectory launch: unknown flags " + std::to_string(L.flags));
+  a.flags = L.flags;
   for (uint32_t j = 0; j < 2 * L.n_pairs; ++j)
     a.pairs[j] = L.pairs[j];
   const bool occ4 = L.waves_per_simd == 4;

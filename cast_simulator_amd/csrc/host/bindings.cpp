@@ -456,7 +456,7 @@ PYBIND11_MODULE(_host, m) {
            py::arg("min_qubits") = 22, py::arg("samples") = 2)
       .def("autotune", &HipKernelManager::autotune)
       .def("tune", &HipKernelManager::tune, py::arg("ids"), py::arg("sv"),
-           py::arg("max_rounds") = 8)
+           py::arg("max_rounds") = 16)
       .def("tuning_report", [](const HipKernelManager &m) {
         py::list out;
         for (const auto &e : m.tuning_report())
@@ -589,7 +589,8 @@ PYBIND11_MODULE(_host, m) {
 
   py::class_<TrajectoryTuning>(m, "TrajectoryTuning")
       .def(py::init([](uint32_t chunk_layers, uint32_t fusion_max, uint32_t block,
-                       uint32_t slots_per_cu, uint32_t max_slots, uint32_t waves_per_simd) {
+                       uint32_t slots_per_cu, uint32_t max_slots, uint32_t waves_per_simd,
+                       uint32_t mfma_f32) {
              TrajectoryTuning t;
              t.chunk_layers = chunk_layers;
              t.fusion_max = fusion_max;
@@ -597,11 +598,14 @@ PYBIND11_MODULE(_host, m) {
              t.slots_per_cu = slots_per_cu;
              t.max_slots = max_slots;
              t.waves_per_simd = waves_per_simd;
+             t.mfma_f32 = mfma_f32;
              return t;
            }),
            py::arg("chunk_layers") = 4, py::arg("fusion_max") = 4, py::arg("block") = 256,
-           py::arg("slots_per_cu") = 2, py::arg("max_slots") = 0, py::arg("waves_per_simd") = 2)
+           py::arg("slots_per_cu") = 2, py::arg("max_slots") = 0, py::arg("waves_per_simd") = 2,
+           py::arg("mfma_f32") = 0)
       .def_readwrite("waves_per_simd", &TrajectoryTuning::waves_per_simd)
+      .def_readwrite("mfma_f32", &TrajectoryTuning::mfma_f32)
       .def_readwrite("chunk_layers", &TrajectoryTuning::chunk_layers)
       .def_readwrite("fusion_max", &TrajectoryTuning::fusion_max)
       .def_readwrite("block", &TrajectoryTuning::block)
@@ -613,7 +617,8 @@ PYBIND11_MODULE(_host, m) {
                ", block=" + std::to_string(t.block) +
                ", slots_per_cu=" + std::to_string(t.slots_per_cu) +
                ", max_slots=" + std::to_string(t.max_slots) +
-               ", waves_per_simd=" + std::to_string(t.waves_per_simd) + ")";
+               ", waves_per_simd=" + std::to_string(t.waves_per_simd) +
+               ", mfma_f32=" + std::to_string(t.mfma_f32) + ")";
       });
 
   py::class_<TrajectoryRun>(m, "TrajectoryRun")
@@ -640,6 +645,7 @@ PYBIND11_MODULE(_host, m) {
       .def_readonly("chunk_layers", &TrajectoryRun::chunk_layers)
       .def_readonly("fusion_max", &TrajectoryRun::fusion_max)
       .def_readonly("waves_per_simd", &TrajectoryRun::waves_per_simd)
+      .def_readonly("mfma_f32", &TrajectoryRun::mfma_f32)
       .def("dump", [](const TrajectoryRun &r) {
         return py::array_t<double>(py::ssize_t(r.dump.size()), r.dump.data());
       });
@@ -746,7 +752,9 @@ PYBIND11_MODULE(_host, m) {
       .def_readwrite("reserve_bytes", &ServeConfig::reserve_bytes)
       .def_readwrite("realtime", &ServeConfig::realtime)
       .def_readwrite("keep_states", &ServeConfig::keep_states)
-      .def_readwrite("f64_shadow", &ServeConfig::f64_shadow);
+      .def_readwrite("f64_shadow", &ServeConfig::f64_shadow)
+      .def_readwrite("batch_max_qubits", &ServeConfig::batch_max_qubits)
+      .def_readwrite("batch_size", &ServeConfig::batch_size);
 
   py::class_<ServeResult>(m, "ServeResult")
       .def_readonly("id", &ServeResult::id)
@@ -782,7 +790,9 @@ PYBIND11_MODULE(_host, m) {
                    py::arg("max_in_flight") = st.max_in_flight,
                    py::arg("peak_sv_bytes") = st.peak_sv_bytes,
                    py::arg("budget_bytes") = st.budget_bytes, py::arg("sv_allocs") = st.sv_allocs,
-                   py::arg("sv_reuses") = st.sv_reuses);
+                   py::arg("sv_reuses") = st.sv_reuses, py::arg("batches") = st.batches,
+                   py::arg("batched_requests") = st.batched_requests,
+                   py::arg("batch_launches") = st.batch_launches);
         return py::make_tuple(res, d);
       });
 }

@@ -112,6 +112,9 @@ constexpr uint32_t kTileGaussSwizzled = 7;
 // ... and both wave-tile forms in XCD-blocked order (CAST_HIP_FLAG_TILE_XCD).
 constexpr uint32_t kTileXcd = 8;
 constexpr uint32_t kTileGaussXcd = 9;
+// ... and of the matrix-core kernel with f32 products for f32 k = 5, 6
+// (CAST_HIP_FLAG_MFMA_F32), whose default there is the bf16 split products.
+constexpr uint32_t kMfmaF32 = 10;
 
 struct TuneEntry {
   KernelId id = 0;
@@ -146,7 +149,7 @@ public:
   bool autotune() const { return autotune_; }
   // Apply `ids` to `sv` repeatedly until every (id, n) is tuned (at most
   // max_rounds passes).  Modifies sv; returns the passes run.
-  uint32_t tune(const std::vector<KernelId> &ids, HipStatevector &sv, uint32_t max_rounds = 8);
+  uint32_t tune(const std::vector<KernelId> &ids, HipStatevector &sv, uint32_t max_rounds = 16);
   std::vector<TuneEntry> tuning_report() const;
 
 private:

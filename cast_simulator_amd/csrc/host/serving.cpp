@@ -104,20 +104,26 @@ std::vector<ServeRequest> poisson_requests(uint32_t count, double rate, uint32_t
   return out;
 }
 
-// One stream slot: a non-blocking stream with its completion event, the
-// reduction scratch and result buffers of the request it is running.
-struct Server::Slot {
-  void *stream = nullptr, *event = nullptr;
-  uint64_t scratch = 0, out = 0;
-  double *h_out = nullptr;
-  void *h_state = nullptr;
-  size_t h_state_bytes = 0;
-  // current request
-  bool busy = false;
+// One admitted request: its statevector (and F64 shadow), gate plans and,
+// with keep_states, a pinned host copy of its final state.
+struct Member {
   size_t req = 0;
   uint64_t sv = 0, shadow = 0; // shadow: F64 replay of an F32 request, or 0
   size_t sv_bytes = 0;
   std::vector<void *> plans, shadow_plans;
+};
+
+// One stream slot: a non-blocking stream with its completion event, the
+// reduction scratch and result buffers of the request (or, in the batch
+// lane, the requests) it is running.
+struct Server::Slot {
+  void *stream = nullptr, *event = nullptr;
+  uint64_t scratch = 0, out = 0; // out: kOut doubles per member
+  double *h_out = nullptr;
+  std::vector<std::pair<void *, size_t>> h_states; // keep_states: per member, reused
+  bool busy = false;
+  bool batched = false;
+  std::vector<Member> members;
 };
 
 // Statevector blocks recycled by exact size under a byte budget.
@@ -184,8 +190,13 @@ Server::Server(ServeConfig cfg) : cfg_(cfg), pool_(new Pool) {
     throw Error("serving: compile_threads must be in 1..64");
   if (cfg_.fusion_max < 1 || cfg_.fusion_max > 6)
     throw Error("serving: fusion_max must be in 1..6");
+  if (cfg_.batch_size < 1 || cfg_.batch_size > 64)
+    throw Error("serving: batch_size must be in 1..64");
+  if (cfg_.batch_max_qubits > 24)
+    throw Error("serving: batch_max_qubits must be at most 24 (0 = no batch lane)");
   char err[kErr];
   const size_t scratch = cast_hip_observe_scratch_bytes(kPairs);
+  const size_t out_doubles = size_t(kOut) * (cfg_.batch_max_qubits ? cfg_.batch_size : 1);
   for (uint32_t i = 0; i < cfg_.streams; ++i) {
     auto *s = new Slot;
     slots_.push_back(s);
@@ -197,9 +208,9 @@ Server::Server(ServeConfig cfg) : cfg_(cfg), pool_(new Pool) {
       throw Error(err);
     s->scratch = cast_hip_malloc(std::max(scratch, cast_hip_inner_product_scratch_bytes()), err,
                                  kErr);
-    s->out = s->scratch ? cast_hip_malloc(kOut * sizeof(double), err, kErr) : 0;
+    s->out = s->scratch ? cast_hip_malloc(out_doubles * sizeof(double), err, kErr) : 0;
     s->h_out = static_cast<double *>(
-        s->out ? cast_hip_host_alloc(kOut * sizeof(double), err, kErr) : nullptr);
+        s->out ? cast_hip_host_alloc(out_doubles * sizeof(double), err, kErr) : nullptr);
     if (!s->h_out)
       throw Error(err);
   }
@@ -210,16 +221,19 @@ Server::~Server() {
   for (Slot *s : slots_) {
     if (s->stream)
       (void)cast_hip_stream_sync(s->stream, err, kErr);
-    for (void *p : s->plans)
-      cast_hip_gate_plan_destroy(p);
-    for (void *p : s->shadow_plans)
-      cast_hip_gate_plan_destroy(p);
-    if (s->busy && s->sv)
-      cast_hip_free(s->sv);
+    for (Member &m : s->members) {
+      for (void *p : m.plans)
+        cast_hip_gate_plan_destroy(p);
+      for (void *p : m.shadow_plans)
+        cast_hip_gate_plan_destroy(p);
+      if (s->busy && m.sv)
+        cast_hip_free(m.sv);
+    }
     cast_hip_free(s->scratch);
     cast_hip_free(s->out);
     cast_hip_host_free(s->h_out);
-    cast_hip_host_free(s->h_state);
+    for (auto &h : s->h_states)
+      cast_hip_host_free(h.first);
     if (s->event)
       cast_hip_event_destroy(s->event);
     if (s->stream)
@@ -351,117 +365,204 @@ std::vector<ServeResult> Server::run(const std::vector<ServeRequest> &reqs, Serv
   // ── scheduler ───────────────────────────────────────────────────────────
   size_t done = 0;
   uint32_t in_flight = 0, max_in_flight = 0;
+  uint64_t n_batches = 0, n_batched = 0, n_batch_launches = 0;
   std::string failure;
-  auto finish = [&](Slot &s) {
-    ServeResult &o = res[s.req];
-    const ServeRequest &r = reqs[s.req];
-    o.done_s = now_s() - t0;
-    o.norm_sq = s.h_out[0];
-    const uint32_t np = r.n_qubits - 1;
-    o.zz.clear();
-    for (uint32_t c = 0; c * kPairs < np; ++c) {
-      const double *h = s.h_out + c * (1 + kPairs) + 1;
-      o.zz.insert(o.zz.end(), h, h + std::min(kPairs, np - c * kPairs));
-    }
-    if (s.shadow) {
-      const double re = s.h_out[kOutInner], im = s.h_out[kOutInner + 1];
-      o.fidelity_vs_f64 = (re * re + im * im) / (s.h_out[0] * s.h_out[kOutShadowNorm]);
-    }
-    if (cfg_.keep_states) {
-      const size_t n_el = size_t(2) << r.n_qubits;
-      o.state.resize(n_el);
-      if (r.precision == CAST_PRECISION_F32) {
-        const float *f = static_cast<const float *>(s.h_state);
-        std::copy(f, f + n_el, o.state.begin());
-      } else {
-        std::memcpy(o.state.data(), s.h_state, n_el * sizeof(double));
-      }
-    }
-    for (void *p : s.plans)
-      cast_hip_gate_plan_destroy(p);
-    for (void *p : s.shadow_plans)
-      cast_hip_gate_plan_destroy(p);
-    s.plans.clear();
-    s.shadow_plans.clear();
-    pool.release(s.sv, s.sv_bytes);
-    s.sv = 0;
-    s.shadow = 0;
-    s.busy = false;
-    --in_flight;
-    ++done;
+  auto is_small = [&](size_t i) {
+    return cfg_.batch_max_qubits != 0 && reqs[i].n_qubits <= cfg_.batch_max_qubits;
   };
-  auto admit = [&](Slot &s, size_t i, Compiled c) {
-    const ServeRequest &r = reqs[i];
-    const size_t b = sv_bytes_of(r.n_qubits, r.precision);
-    const bool shadow = !c.shadow_plans.empty();
-    const size_t held = request_bytes(r, shadow);
-    s.sv = pool.acquire(held);
-    if (!s.sv)
-      throw Error("serving: statevector pool refused an admitted request");
-    s.sv_bytes = held;
-    s.shadow = shadow ? s.sv + round_up_256(b) : 0;
-    s.req = i;
-    s.plans = std::move(c.plans);
-    s.shadow_plans = std::move(c.shadow_plans);
-    s.busy = true;
-    ++in_flight;
-    max_in_flight = std::max(max_in_flight, in_flight);
-    ServeResult &o = res[i];
-    o.id = r.id;
-    o.n_qubits = r.n_qubits;
-    o.depth = r.depth;
-    o.precision = r.precision;
-    o.arrival_s = r.arrival_s;
-    o.n_gates = uint32_t(s.plans.size());
-    o.admit_s = now_s() - t0;
-    const uint8_t prec = uint8_t(r.precision);
-    throw_if(cast_hip_sv_zero_async(s.sv, size_t(2) << r.n_qubits, prec, s.stream, err, kErr),
-             err);
-    for (void *p : s.plans)
-      throw_if(cast_hip_gate_apply(p, s.stream, s.sv, r.n_qubits, err, kErr), err);
-    // Norm and neighbour pairs in chunks of kPairs (the scratch is reused in
-    // stream order).
-    const uint32_t np = r.n_qubits - 1;
-    uint32_t pairs[2 * kPairs];
-    for (uint32_t c = 0; c == 0 || c * kPairs < np; ++c) {
-      const uint32_t m = std::min(kPairs, np - c * kPairs);
-      for (uint32_t j = 0; j < m; ++j) {
-        pairs[2 * j] = c * kPairs + j;
-        pairs[2 * j + 1] = c * kPairs + j + 1;
+  auto release_plans = [](Member &m) {
+    for (void *p : m.plans)
+      cast_hip_gate_plan_destroy(p);
+    for (void *p : m.shadow_plans)
+      cast_hip_gate_plan_destroy(p);
+    m.plans.clear();
+    m.shadow_plans.clear();
+  };
+  auto finish = [&](Slot &s) {
+    const double t_done = now_s() - t0;
+    for (size_t mi = 0; mi < s.members.size(); ++mi) {
+      Member &m = s.members[mi];
+      ServeResult &o = res[m.req];
+      const ServeRequest &r = reqs[m.req];
+      const double *h = s.h_out + mi * kOut;
+      o.done_s = t_done;
+      o.norm_sq = h[0];
+      const uint32_t np = r.n_qubits - 1;
+      o.zz.clear();
+      for (uint32_t c = 0; c * kPairs < np; ++c) {
+        const double *z = h + c * (1 + kPairs) + 1;
+        o.zz.insert(o.zz.end(), z, z + std::min(kPairs, np - c * kPairs));
       }
-      throw_if(cast_hip_observe_async(s.sv, r.n_qubits, prec, pairs, m,
-                                      s.out + c * (1 + kPairs) * sizeof(double), s.scratch,
-                                      s.stream, err, kErr),
-               err);
+      if (m.shadow) {
+        const double re = h[kOutInner], im = h[kOutInner + 1];
+        o.fidelity_vs_f64 = (re * re + im * im) / (h[0] * h[kOutShadowNorm]);
+      }
+      if (cfg_.keep_states) {
+        const size_t n_el = size_t(2) << r.n_qubits;
+        o.state.resize(n_el);
+        const void *hs = s.h_states[mi].first;
+        if (r.precision == CAST_PRECISION_F32) {
+          const float *f = static_cast<const float *>(hs);
+          std::copy(f, f + n_el, o.state.begin());
+        } else {
+          std::memcpy(o.state.data(), hs, n_el * sizeof(double));
+        }
+      }
+      release_plans(m);
+      pool.release(m.sv, m.sv_bytes);
+      --in_flight;
+      ++done;
     }
-    if (s.shadow) {
+    s.members.clear();
+    s.busy = false;
+  };
+  // Queue a slot's members (statevectors already acquired): |0> fill, the
+  // gates (one launch per gate, or in the batch lane one launch per step and
+  // (precision, k) group), the reductions, the result copy and the event.
+  auto admit = [&](Slot &s, bool batched) {
+    s.busy = true;
+    s.batched = batched;
+    const double t_admit = now_s() - t0;
+    for (Member &m : s.members) {
+      const ServeRequest &r = reqs[m.req];
+      ServeResult &o = res[m.req];
+      o.id = r.id;
+      o.n_qubits = r.n_qubits;
+      o.depth = r.depth;
+      o.precision = r.precision;
+      o.arrival_s = r.arrival_s;
+      o.n_gates = uint32_t(m.plans.size());
+      o.admit_s = t_admit;
+      ++in_flight;
       const size_t n_el = size_t(2) << r.n_qubits;
-      throw_if(cast_hip_sv_zero_async(s.shadow, n_el, 1, s.stream, err, kErr), err);
-      for (void *p : s.shadow_plans)
-        throw_if(cast_hip_gate_apply(p, s.stream, s.shadow, r.n_qubits, err, kErr), err);
-      throw_if(cast_hip_inner_product_async(s.sv, prec, s.shadow, 1, r.n_qubits,
-                                            s.out + kOutInner * sizeof(double), s.scratch,
-                                            s.stream, err, kErr),
-               err);
-      throw_if(cast_hip_observe_async(s.shadow, r.n_qubits, 1, pairs, 0,
-                                      s.out + kOutShadowNorm * sizeof(double), s.scratch,
-                                      s.stream, err, kErr),
-               err);
+      throw_if(cast_hip_sv_zero_async(m.sv, n_el, uint8_t(r.precision), s.stream, err, kErr), err);
+      if (m.shadow)
+        throw_if(cast_hip_sv_zero_async(m.shadow, n_el, 1, s.stream, err, kErr), err);
     }
-    throw_if(cast_hip_memcpy_dtoh_async(s.h_out, s.out, kOut * sizeof(double), s.stream, err,
-                                        kErr),
+    max_in_flight = std::max(max_in_flight, in_flight);
+    if (!batched) {
+      for (Member &m : s.members) {
+        const uint32_t n = reqs[m.req].n_qubits;
+        for (void *p : m.plans)
+          throw_if(cast_hip_gate_apply(p, s.stream, m.sv, n, err, kErr), err);
+        for (void *p : m.shadow_plans)
+          throw_if(cast_hip_gate_apply(p, s.stream, m.shadow, n, err, kErr), err);
+      }
+    } else {
+      ++n_batches;
+      n_batched += s.members.size();
+      size_t steps = 0;
+      for (const Member &m : s.members)
+        steps = std::max({steps, m.plans.size(), m.shadow_plans.size()});
+      struct Group {
+        std::vector<const void *> plans;
+        std::vector<uint64_t> svs;
+        std::vector<uint32_t> ns;
+      };
+      auto plan_key = [&](void *p) {
+        cast_hip_gate_plan_info_t info{};
+        throw_if(cast_hip_gate_plan_info(p, &info, err, kErr), err);
+        return std::make_pair(uint32_t(info.precision), info.n_qubits);
+      };
+      for (size_t step = 0; step < steps; ++step) {
+        std::map<std::pair<uint32_t, uint32_t>, Group> groups; // (precision, k)
+        for (const Member &m : s.members) {
+          const uint32_t n = reqs[m.req].n_qubits;
+          if (step < m.plans.size()) {
+            Group &g = groups[plan_key(m.plans[step])];
+            g.plans.push_back(m.plans[step]);
+            g.svs.push_back(m.sv);
+            g.ns.push_back(n);
+          }
+          if (step < m.shadow_plans.size()) {
+            Group &g = groups[plan_key(m.shadow_plans[step])];
+            g.plans.push_back(m.shadow_plans[step]);
+            g.svs.push_back(m.shadow);
+            g.ns.push_back(n);
+          }
+        }
+        for (auto &kv : groups) {
+          Group &g = kv.second;
+          throw_if(cast_hip_gate_apply_batch(g.plans.data(), g.svs.data(), g.ns.data(),
+                                             g.plans.size(), s.stream, err, kErr),
+                   err);
+          n_batch_launches += (g.plans.size() + 63) / 64;
+        }
+      }
+    }
+    // Norm and neighbour pairs in chunks of kPairs, per member (the scratch
+    // is reused in stream order).
+    for (size_t mi = 0; mi < s.members.size(); ++mi) {
+      const Member &m = s.members[mi];
+      const ServeRequest &r = reqs[m.req];
+      const uint8_t prec = uint8_t(r.precision);
+      const uint64_t out = s.out + mi * kOut * sizeof(double);
+      const uint32_t np = r.n_qubits - 1;
+      uint32_t pairs[2 * kPairs];
+      for (uint32_t c = 0; c == 0 || c * kPairs < np; ++c) {
+        const uint32_t cnt = std::min(kPairs, np - c * kPairs);
+        for (uint32_t j = 0; j < cnt; ++j) {
+          pairs[2 * j] = c * kPairs + j;
+          pairs[2 * j + 1] = c * kPairs + j + 1;
+        }
+        throw_if(cast_hip_observe_async(m.sv, r.n_qubits, prec, pairs, cnt,
+                                        out + c * (1 + kPairs) * sizeof(double), s.scratch,
+                                        s.stream, err, kErr),
+                 err);
+      }
+      if (m.shadow) {
+        throw_if(cast_hip_inner_product_async(m.sv, prec, m.shadow, 1, r.n_qubits,
+                                              out + kOutInner * sizeof(double), s.scratch,
+                                              s.stream, err, kErr),
+                 err);
+        throw_if(cast_hip_observe_async(m.shadow, r.n_qubits, 1, pairs, 0,
+                                        out + kOutShadowNorm * sizeof(double), s.scratch,
+                                        s.stream, err, kErr),
+                 err);
+      }
+    }
+    throw_if(cast_hip_memcpy_dtoh_async(s.h_out, s.out, s.members.size() * kOut * sizeof(double),
+                                        s.stream, err, kErr),
              err);
     if (cfg_.keep_states) {
-      if (s.h_state_bytes < b) {
-        cast_hip_host_free(s.h_state);
-        s.h_state = cast_hip_host_alloc(b, err, kErr);
-        if (!s.h_state)
-          throw Error(err);
-        s.h_state_bytes = b;
+      if (s.h_states.size() < s.members.size())
+        s.h_states.resize(s.members.size(), {nullptr, 0});
+      for (size_t mi = 0; mi < s.members.size(); ++mi) {
+        const ServeRequest &r = reqs[s.members[mi].req];
+        const size_t b = sv_bytes_of(r.n_qubits, r.precision);
+        auto &h = s.h_states[mi];
+        if (h.second < b) {
+          cast_hip_host_free(h.first);
+          h.first = cast_hip_host_alloc(b, err, kErr);
+          if (!h.first) {
+            h.second = 0;
+            throw Error(err);
+          }
+          h.second = b;
+        }
+        throw_if(cast_hip_memcpy_dtoh_async(h.first, s.members[mi].sv, b, s.stream, err, kErr),
+                 err);
       }
-      throw_if(cast_hip_memcpy_dtoh_async(s.h_state, s.sv, b, s.stream, err, kErr), err);
     }
     throw_if(cast_hip_event_record(s.event, s.stream, err, kErr), err);
+  };
+  // Statevector memory for request i (and its shadow), or 0 if it does not
+  // fit the budget yet.
+  auto take = [&](size_t i, Compiled &c, Member &m) {
+    const ServeRequest &r = reqs[i];
+    const bool shadow = !c.shadow_plans.empty();
+    const size_t held = request_bytes(r, shadow);
+    if (!pool.fits(held))
+      return false;
+    m.sv = pool.acquire(held);
+    if (!m.sv)
+      return false;
+    m.req = i;
+    m.sv_bytes = held;
+    m.shadow = shadow ? m.sv + round_up_256(sv_bytes_of(r.n_qubits, r.precision)) : 0;
+    m.plans = std::move(c.plans);
+    m.shadow_plans = std::move(c.shadow_plans);
+    return true;
   };
 
   try {
@@ -481,27 +582,39 @@ std::vector<ServeResult> Server::run(const std::vector<ServeRequest> &reqs, Serv
         if (s->busy)
           continue;
         // Oldest compiled request that fits; younger ones may pass an older
-        // one only when the older one's statevector does not fit yet.
-        size_t pick = SIZE_MAX;
-        Compiled plans;
+        // one only when the older one's statevector does not fit yet.  A
+        // small request takes up to batch_size - 1 further small ones along
+        // (oldest first, each only if it fits).
+        bool batched = false;
         {
           std::lock_guard<std::mutex> lock(mu);
-          for (auto it = ready.begin(); it != ready.end(); ++it) {
+          for (auto it = ready.begin(); it != ready.end();) {
             const size_t i = it->second;
             if (!compiled[i].error.empty())
               throw Error("serving: compiling request " + std::to_string(reqs[i].id) + ": " +
                           compiled[i].error);
-            if (pool.fits(request_bytes(reqs[i], !compiled[i].shadow_plans.empty()))) {
-              pick = i;
-              plans = std::move(compiled[i]);
-              ready.erase(it);
-              break;
+            const bool small = is_small(i);
+            if (!s->members.empty() && (!batched || !small)) {
+              ++it;
+              continue;
             }
+            Member m;
+            if (!take(i, compiled[i], m)) {
+              ++it;
+              continue;
+            }
+            s->members.push_back(std::move(m));
+            it = ready.erase(it);
+            if (!small)
+              break;
+            batched = true;
+            if (s->members.size() >= cfg_.batch_size)
+              break;
           }
         }
-        if (pick == SIZE_MAX)
+        if (s->members.empty())
           break;
-        admit(*s, pick, std::move(plans));
+        admit(*s, batched);
         progress = true;
       }
       if (!progress) {
@@ -519,16 +632,12 @@ std::vector<ServeResult> Server::run(const std::vector<ServeRequest> &reqs, Serv
     // Drain the device, then release everything this run still holds.
     for (Slot *s : slots_) {
       (void)cast_hip_stream_sync(s->stream, err, kErr);
-      if (s->busy) {
-        for (void *p : s->plans)
-          cast_hip_gate_plan_destroy(p);
-        for (void *p : s->shadow_plans)
-          cast_hip_gate_plan_destroy(p);
-        s->plans.clear();
-        s->shadow_plans.clear();
-        pool.release(s->sv, s->sv_bytes);
-        s->busy = false;
+      for (Member &m : s->members) {
+        release_plans(m);
+        pool.release(m.sv, m.sv_bytes);
       }
+      s->members.clear();
+      s->busy = false;
     }
     for (auto &c : compiled) {
       for (void *p : c.plans)
@@ -562,6 +671,9 @@ std::vector<ServeResult> Server::run(const std::vector<ServeRequest> &reqs, Serv
     st.budget_bytes = pool.budget;
     st.sv_allocs = pool.allocs;
     st.sv_reuses = pool.reuses;
+    st.batches = n_batches;
+    st.batched_requests = n_batched;
+    st.batch_launches = n_batch_launches;
   }
   return res;
 }

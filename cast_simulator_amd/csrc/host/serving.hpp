@@ -67,6 +67,14 @@ struct ServeConfig {
   // doubles the memory and adds the F64 work of each F32 request, so it is
   // for accuracy runs, not for latency figures.
   bool f64_shadow = false;
+  // Batch lane: requests of at most batch_max_qubits qubits (0 = off) are
+  // admitted up to batch_size (1..64) at a time into one stream slot, and
+  // each fused-gate step of the batch is one launch per (precision, gate
+  // size) through cast_hip_gate_apply_batch instead of one launch per gate
+  // per request.  Such a request always runs this way (alone too), so its
+  // result does not depend on the mix.
+  uint32_t batch_max_qubits = 18;
+  uint32_t batch_size = 32;
 };
 
 struct ServeResult {
@@ -88,6 +96,7 @@ struct ServeStats {
   uint64_t peak_sv_bytes = 0;
   uint64_t budget_bytes = 0;
   uint64_t sv_allocs = 0, sv_reuses = 0;
+  uint64_t batches = 0, batched_requests = 0, batch_launches = 0; // batch lane
 };
 
 class Server {

@@ -635,6 +635,7 @@ TrajectoryRun TrajectoryEngine::run(const std::vector<TrajectorySample> &samples
   R.n_slots = uint32_t(n_slots);
   R.block = tn.block;
   R.waves_per_simd = tn.waves_per_simd;
+  R.mfma_f32 = tn.mfma_f32;
 
   slots_.ensure(n_slots * svb);
   pool_.ensure(std::max<size_t>(16, pool.size() * elem));
@@ -689,6 +690,7 @@ TrajectoryRun TrajectoryEngine::run(const std::vector<TrajectorySample> &samples
   L.dump = dump ? dump_.ptr : 0;
   L.work = work_.ptr;
   L.waves_per_simd = tn.waves_per_simd;
+  L.flags = tn.mfma_f32 ? CAST_HIP_TRAJ_FLAG_MFMA_F32 : 0u;
 
   void *e0 = cast_hip_event_create(err, kErr);
   if (!e0)
@@ -766,8 +768,8 @@ TrajectoryRun TrajectoryEngine::run(const std::vector<TrajectorySample> &samples
 TrajectoryTuning TrajectoryEngine::autotune(const std::vector<TrajectorySample> &sample,
                                             const TrajectoryTuning &start,
                                             std::vector<TrajectoryTuneRecord> *log) {
-  TrajectoryTuning best = start;
-  double best_s = 1e300;
+  TrajectoryTuning best = start, second = start;
+  double best_s = 1e300, second_s = 1e300;
   auto trial = [&](const TrajectoryTuning &t) {
     // Checkpoints are part of the cost of a chunk size: time them too.
     ckpt_chunk_ = 0;
@@ -776,8 +778,13 @@ TrajectoryTuning TrajectoryEngine::autotune(const std::vector<TrajectorySample> 
     if (log)
       log->push_back({t, s});
     if (s < best_s) {
+      second_s = best_s;
+      second = best;
       best_s = s;
       best = t;
+    } else if (s < second_s) {
+      second_s = s;
+      second = t;
     }
   };
   trial(best);
@@ -814,6 +821,24 @@ TrajectoryTuning TrajectoryEngine::autotune(const std::vector<TrajectorySample> 
         continue;
       trial(t);
     }
+  // f32 k = 5, 6 gates: the other matrix-core product form (bf16 split or
+  // f32 products) at the two best settings so far.
+  if (prec_ == CAST_PRECISION_F32) {
+    auto same = [](const TrajectoryTuning &a, const TrajectoryTuning &b) {
+      return a.chunk_layers == b.chunk_layers && a.fusion_max == b.fusion_max &&
+             a.block == b.block && a.slots_per_cu == b.slots_per_cu &&
+             a.waves_per_simd == b.waves_per_simd && a.mfma_f32 == b.mfma_f32;
+    };
+    const TrajectoryTuning top[2] = {best, second};
+    for (int i = 0; i < 2; ++i) {
+      const TrajectoryTuning &b = top[i];
+      if (b.fusion_max < 5 || (i == 1 && same(top[0], top[1])))
+        continue;
+      TrajectoryTuning t = b;
+      t.mfma_f32 = b.mfma_f32 ? 0u : 1u;
+      trial(t);
+    }
+  }
   return best;
 }
 

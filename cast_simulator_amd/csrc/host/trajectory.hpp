@@ -103,6 +103,7 @@ struct TrajectoryTuning {
   uint32_t slots_per_cu = 2; // resident statevector slots per compute unit
   uint32_t max_slots = 0;    // 0 = no cap beyond memory and slots_per_cu
   uint32_t waves_per_simd = 2; // kernel build: 2 (no spills) or 4 (two workgroups per CU)
+  uint32_t mfma_f32 = 0;       // f32 k = 5, 6 gates: 0 = bf16 split products, 1 = f32 products
 };
 
 struct TrajectoryRun {
@@ -122,6 +123,7 @@ struct TrajectoryRun {
   double plan_s = 0.0, checkpoint_s = 0.0, gpu_s = 0.0, total_s = 0.0;
   uint64_t n_tasks = 0, n_gate_passes = 0, n_shared_clean = 0, n_dirty_matrices = 0;
   uint32_t n_slots = 0, block = 0, chunk_layers = 0, fusion_max = 0, waves_per_simd = 0;
+  uint32_t mfma_f32 = 0;
   std::vector<double> dump; // final states per task (interleaved f64), if asked for
 };
 

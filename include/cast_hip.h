@@ -82,6 +82,13 @@ typedef struct cast_hip_kernel_gen_spec_t {
  * instead of every XCD interleaving with the others.  Faster for some
  * target sets and slower for others; the host autotuner tries it. */
 #define CAST_HIP_FLAG_TILE_XCD 4u
+/* Dense k = 6 at F32 on the matrix cores: through bf16 split products
+ * (every f32 operand split into three bf16 parts, six partial products per
+ * real product on v_mfma_f32_16x16x32_bf16, f32 accumulation), or on the
+ * f32 matrix cores (v_mfma_f32_16x16x4_f32).  At most one of the two; with
+ * neither the library picks.  Ignored elsewhere. */
+#define CAST_HIP_FLAG_MFMA_SPLIT 8u
+#define CAST_HIP_FLAG_MFMA_F32 16u
 
 /* AUTO: monomial when every row has <= 1 nonzero, else the dense family, whose
  * kernel (direct / wave tile / matrix core) is picked per launch from k, the
@@ -365,8 +372,12 @@ typedef struct cast_hip_traj_launch_t {
   uint32_t waves_per_simd;/* occupancy the kernel is built for: 2 (256 VGPRs, */
                           /* no spills) or 4 (128 VGPRs, two 512-thread     */
                           /* workgroups per CU); 0 = 2                      */
-  uint32_t reserved;      /* 0                                            */
+  uint32_t flags;         /* CAST_HIP_TRAJ_FLAG_*, 0 = defaults          */
 } cast_hip_traj_launch_t;
+
+/* k = 5, 6 gates (F32) on the f32 matrix cores (v_mfma_f32_16x16x4_f32)
+ * instead of the default bf16 split products (see CAST_HIP_FLAG_MFMA_SPLIT). */
+#define CAST_HIP_TRAJ_FLAG_MFMA_F32 1u
 
 /* Enqueue the persistent launch on `stream` (returns without waiting).  A
  * gate descriptor that is invalid on the device (k outside 1..4 at F64 or

@@ -192,12 +192,13 @@ VARIANTS = [(abi.VARIANT_AUTO, 0), (abi.VARIANT_DENSE, 0), (abi.VARIANT_TILE, 0)
             (abi.VARIANT_TILE, abi.FLAG_TILE_GAUSS | abi.FLAG_TILE_SWIZZLE),
             (abi.VARIANT_TILE, abi.FLAG_TILE_XCD),
             (abi.VARIANT_TILE, abi.FLAG_TILE_GAUSS | abi.FLAG_TILE_XCD),
-            (abi.VARIANT_MFMA, 0)]
+            (abi.VARIANT_MFMA, 0), (abi.VARIANT_MFMA, abi.FLAG_MFMA_F32)]
 
 
 @pytest.mark.parametrize("variant,flags", VARIANTS,
                          ids=["auto", "direct", "tile", "tile-swizzled", "tile-gauss",
-                              "tile-gauss-swizzled", "tile-xcd", "tile-gauss-xcd", "mfma"])
+                              "tile-gauss-swizzled", "tile-xcd", "tile-gauss-xcd", "mfma",
+                              "mfma-f32-products"])
 @pytest.mark.parametrize("f32", [False, True], ids=["f64", "f32"])
 def test_every_kernel_family_every_shape(f32, variant, flags):
     """Each dense kernel family, forced through the C ABI, on every target
@@ -442,9 +443,14 @@ def test_autotuned_kernels_match_oracle(f32):
     assert mgr.tune(ids, scratch) >= 1
     report = mgr.tuning_report()
     # chosen: direct (1), wave tile (3), matrix core (4), wave tile swizzled (5),
-    # three products (6), both (7), XCD order (8), three products in XCD order (9)
-    assert len(report) == len(ids) and all(e["chosen"] in (1, 3, 4, 5, 6, 7, 8, 9)
+    # three products (6), both (7), XCD order (8), three products in XCD order
+    # (9), matrix core with f32 products at f32 k >= 5 (10)
+    assert len(report) == len(ids) and all(e["chosen"] in (1, 3, 4, 5, 6, 7, 8, 9, 10)
                                            for e in report)
+    if f32:  # both matrix-core product forms are candidates for the k = 5 gates
+        for e in report:
+            codes = [c for c, _ in e["best_s"]]
+            assert (4 in codes and 10 in codes) == (len(gates[e["id"]].qubits()) == 5)
     init = np.asarray(cs.random_statevector(n, 3))
     sv = cs.HipStatevector(n, prec(f32))
     sv.upload(init)
@@ -482,7 +488,8 @@ ROUNDTRIP_SHAPES = [[0, 1, 2, 3, 4], [5, 11, 17, 23, 29], [1, 9, 15, 26], [3, 6,
 
 @pytest.mark.parametrize("variant,flags", VARIANTS[1:],
                          ids=["direct", "tile", "tile-swizzled", "tile-gauss",
-                              "tile-gauss-swizzled", "tile-xcd", "tile-gauss-xcd", "mfma"])
+                              "tile-gauss-swizzled", "tile-xcd", "tile-gauss-xcd", "mfma",
+                              "mfma-f32-products"])
 @pytest.mark.parametrize("f32", [False, True], ids=["f64", "f32"])
 def test_roundtrip_at_30_qubits_every_family(f32, variant, flags):
     """Each kernel family forced through the C ABI at the headline size: the
@@ -554,3 +561,24 @@ def test_inner_product_and_fidelity_match_numpy(pa, pb):
     assert abs(a.fidelity(a) - 1.0) < 1e-12
     with pytest.raises(cs.CastError):
         a.inner(cs.HipStatevector(n - 1, prec[pb]))
+
+
+@pytest.mark.parametrize("qubits", [[0, 1, 2, 3, 4], [2, 5, 9, 13, 15], [11, 12, 13, 14, 15],
+                                    [0, 1, 2, 3, 4, 5], [2, 5, 9, 11, 13, 15],
+                                    [10, 11, 12, 13, 14, 15], [0, 3, 7, 8, 12, 13]])
+def test_f32_split_products_accuracy(qubits):
+    """f32 k = 5, 6 on the matrix cores: the bf16 split products (every f32
+    operand in three bf16 parts, six partial products) against the f64 oracle
+    on the same f32-rounded input: relative L2 error within 1.5e-7 (f32
+    output rounding is about 6e-8), and no larger than the f32 products'."""
+    n = 16
+    g = cs.QuantumGate.random_unitary(qubits, 900 + sum(qubits))
+    init = seeded_state(n).astype(np.float32).astype(np.float64)
+    want = apply_oracle([g], init, False)
+    errs = {}
+    for name, fl in (("split", abi.FLAG_MFMA_SPLIT), ("f32", abi.FLAG_MFMA_F32)):
+        got = apply_abi(g.matrix(), qubits, init, True, abi.VARIANT_MFMA, n, fl)
+        errs[name] = float(np.linalg.norm(got - want) / np.linalg.norm(want))
+    print(f"[parity] split products {qubits}: {errs}", flush=True)
+    assert errs["split"] < 1.5e-7, errs
+    assert errs["split"] <= errs["f32"], errs

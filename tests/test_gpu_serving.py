@@ -16,12 +16,18 @@ from helpers import fidelity, rel_l2
 pytestmark = pytest.mark.gpu
 
 
-def serve(reqs, streams, threads=4, realtime=False, keep=True, budget=0, shadow=False):
+def serve(reqs, streams, threads=4, realtime=False, keep=True, budget=0, shadow=False,
+          batch_size=None, batch_max_qubits=None):
     cfg = cs.ServeConfig()
     cfg.streams, cfg.compile_threads, cfg.realtime, cfg.keep_states = streams, threads, realtime, keep
     cfg.budget_bytes = budget
     cfg.f64_shadow = shadow
-    print(f"[serve] {len(reqs)} requests, streams={streams}, realtime={realtime}", flush=True)
+    if batch_size is not None:
+        cfg.batch_size = batch_size
+    if batch_max_qubits is not None:
+        cfg.batch_max_qubits = batch_max_qubits
+    print(f"[serve] {len(reqs)} requests, streams={streams}, realtime={realtime}, "
+          f"batch {cfg.batch_size} up to {cfg.batch_max_qubits} qubits", flush=True)
     res, st = cs.Server(cfg).run(reqs)
     return {r.id: r for r in res}, st
 
@@ -37,11 +43,21 @@ def oracle_state(r, circuit=None):
     return sv
 
 
-def test_results_are_the_same_alone_and_in_the_mix_and_match_the_oracle():
+@pytest.mark.parametrize("batch_max", [0, 18], ids=["per-request", "batch-lane"])
+def test_results_are_the_same_alone_and_in_the_mix_and_match_the_oracle(batch_max):
+    """Alone: one stream, one request at a time (batches of one in the batch
+    lane); mixed: 8 streams, and batches of up to 32 small requests."""
     reqs = cs.poisson_requests(48, 1000.0, 6, 16, 3, 20, 0.5, 7)
-    alone, st1 = serve(reqs, streams=1, threads=1)
-    mixed, st8 = serve(reqs, streams=8, threads=8)
+    alone, st1 = serve(reqs, streams=1, threads=1, batch_size=1, batch_max_qubits=batch_max)
+    mixed, st8 = serve(reqs, streams=8, threads=8, batch_max_qubits=batch_max)
     assert st1["max_in_flight"] == 1
+    if batch_max:
+        assert st1["batched_requests"] == st8["batched_requests"] == len(reqs)
+        assert st8["batches"] < len(reqs)  # several requests per batch in the mix
+        # one launch per step and gate size, not one per gate per request
+        assert st8["batch_launches"] < sum(r.n_gates for r in mixed.values())
+    else:
+        assert st1["batches"] == st8["batches"] == 0
     for r in reqs:
         a, m = alone[r.id], mixed[r.id]
         assert a.n_gates == m.n_gates and a.n_gates > 0
@@ -117,7 +133,7 @@ def test_circuit_and_qasm_requests_match_the_oracle():
     circuits = {0: qft, 1: qft, 2: prog, 3: prog}
     assert reqs[2].n_qubits == 7 and reqs[0].has_circuit and not extra[0].has_circuit
     res, st = serve(reqs + extra, streams=4, threads=4)
-    assert st["max_in_flight"] <= 4
+    assert st["max_in_flight"] <= 4 * 32  # 4 streams, batches of up to 32 small requests
     for r in reqs + extra:
         o = res[r.id]
         want = oracle_state(r, circuits.get(r.id))
@@ -180,3 +196,26 @@ def test_34_qubit_f32_request_reduces_every_neighbour_pair():
     assert len(o.zz) == n - 1
     assert abs(o.norm_sq - 1.0) < 1e-5
     assert np.allclose(np.asarray(o.zz) / o.norm_sq, want, rtol=0, atol=1e-5)
+
+
+def test_batch_lane_with_f64_shadow_and_large_requests_beside_it():
+    """Small requests (n <= 18) in batches, larger ones one per stream, in one
+    run with the F64 replay of every F32 request: each state matches the
+    oracle and every F32 request has a fidelity >= 1 - 1e-6; the batch lane
+    and the per-request path give the same large-request states bit for bit."""
+    reqs = cs.poisson_requests(40, 1000.0, 10, 20, 4, 12, 0.5, 23)
+    res, st = serve(reqs, streams=4, threads=8, shadow=True)
+    small = [r for r in reqs if r.n_qubits <= 18]
+    assert st["batched_requests"] == len(small) > 0 and st["batches"] >= 1
+    ref, _ = serve([r for r in reqs if r.n_qubits > 18], streams=1, threads=4,
+                   batch_max_qubits=0)
+    for r in reqs:
+        o = res[r.id]
+        tol = 1e-12 if r.precision == cs.Precision.F64 else 1e-5
+        assert rel_l2(oracle_state(r), o.state()) < tol, f"request {r.id}"
+        if r.precision == cs.Precision.F32:
+            assert o.fidelity_vs_f64 >= 1 - 1e-6, (r.id, o.fidelity_vs_f64)
+        else:
+            assert o.fidelity_vs_f64 == -1.0
+        if r.n_qubits > 18:
+            assert np.array_equal(o.state(), ref[r.id].state()), f"request {r.id}"

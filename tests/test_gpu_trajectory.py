@@ -129,11 +129,13 @@ def test_noiseless_limit_equals_statevector_expectations():
     assert abs(r.min_norm_sq - 1.0) < 1e-12 and abs(r.max_norm_sq - 1.0) < 1e-12
 
 
-@pytest.mark.parametrize("kmax32", [4, 6])
-def test_f32_matches_f64_at_24_qubits(kmax32):
+@pytest.mark.parametrize("kmax32,mf32", [(4, 0), (6, 0), (6, 1)],
+                         ids=["k4", "k6-split", "k6-f32products"])
+def test_f32_matches_f64_at_24_qubits(kmax32, mf32):
     """The benchmark's statevector size: f32 trajectories (k <= 4 on wave
-    tiles, or k <= 6 with the k = 5, 6 gates on matrix cores) against the
-    f64 trajectories of the same samples (k <= 4), state by state."""
+    tiles, or k <= 6 with the k = 5, 6 gates on matrix cores, as bf16 split
+    or f32 products) against the f64 trajectories of the same samples
+    (k <= 4), state by state."""
     n, D = 24, 12
     lc = cs.LayeredCircuit.random_layers(n, D, 2026)
     S = cs.sample_trajectories(lc, cs.NoiseSpec(1e-2, 10, D, 4), 6)
@@ -141,7 +143,8 @@ def test_f32_matches_f64_at_24_qubits(kmax32):
     flat = [q for p in pairs for q in p]
     print(f"[traj] n=24 f32 (k<={kmax32}) vs f64 dump, 6 trajectories", flush=True)
     a = cs.TrajectoryEngine(lc, F64, flat).run(S, cs.TrajectoryTuning(4, 4, 256, 2, 0), True)
-    b = cs.TrajectoryEngine(lc, F32, flat).run(S, cs.TrajectoryTuning(4, kmax32, 512, 1, 0), True)
+    b = cs.TrajectoryEngine(lc, F32, flat).run(
+        S, cs.TrajectoryTuning(4, kmax32, 512, 1, 0, 2, mf32), True)
     assert b.n_dirty_matrices > 0
     da, db = a.dump().reshape(a.n_tasks, -1), b.dump().reshape(b.n_tasks, -1)
     for t in range(len(S)):
@@ -330,20 +333,21 @@ def noisy_layers(n, depth, seed, gamma, pxx):
     return cs.LayeredCircuit.from_circuit(qc)
 
 
-@pytest.mark.parametrize("prec,kmax", [(F64, 4), (F32, 4), (F32, 6)])
-def test_noise_model_tasks_match_oracle(prec, kmax):
+@pytest.mark.parametrize("prec,kmax,mf32", [(F64, 4, 0), (F32, 4, 0), (F32, 6, 0), (F32, 6, 1)],
+                         ids=["f64-k4", "f32-k4", "f32-k6-split", "f32-k6-f32products"])
+def test_noise_model_tasks_match_oracle(prec, kmax, mf32):
     """Non-unitary and correlated branches sampled per trajectory (plus
     depolarizing Paulis): every dumped state against the oracle run of the
     trajectory's explicit gate list; norms differ from 1 and are divided out
-    in f64."""
+    in f64.  f32 k = 5, 6 gates run as bf16 split products or f32 products."""
     n = 12
     lc = noisy_layers(n, 5, 61, gamma=0.3, pxx=0.1)
     S = cs.sample_trajectories(lc, cs.NoiseSpec(0.01, 1, lc.n_layers(), 62), 32)
     assert sum(1 for s in S for e in s.errors if e[2] == 0) > 10
     init = np.asarray(cs.random_statevector(n, 63))
     pairs = [(i, i + 1) for i in range(n - 1)]
-    r = check_run(lc, S, prec, cs.TrajectoryTuning(2, kmax, 256, 2, 0), pairs, init)
-    assert r.min_norm_sq < 0.99
+    r = check_run(lc, S, prec, cs.TrajectoryTuning(2, kmax, 256, 2, 0, 2, mf32), pairs, init)
+    assert r.min_norm_sq < 0.99 and r.mfma_f32 == mf32
 
 
 def test_noise_model_mean_matches_density_matrix():

@@ -55,3 +55,5 @@ def test_circuit_requests_take_their_qubit_count_from_the_circuit():
 def test_serve_config_defaults_to_one_stream_per_hardware_queue():
     cfg = cs.ServeConfig()
     assert cfg.streams == 4 and cfg.f64_shadow is False
+    # small requests in batches of up to 32 by default
+    assert cfg.batch_max_qubits == 18 and cfg.batch_size == 32
