@@ -1,6 +1,6 @@
 """Summarise rocprofv3 CSV output of a benchmark run.
 
-    python -m cast_simulator_amd.rocprof_summary kernel_trace.csv [--last N]
+    python -m cast_simulator_amd.rocprof_summary kernel_trace.csv [--last N] [--precision f64|f32]
     python -m cast_simulator_amd.rocprof_summary counter_collection.csv --counters
     python -m cast_simulator_amd.rocprof_summary FETCH.csv --traffic WRITE.csv \
         --copy-bytes B --last N --algorithmic A [--kernels k_traj]
@@ -44,7 +44,17 @@ def _table(rows, title):
     return "\n".join(out) + "\n"
 
 
-def kernel_trace(path: str, last: int | None) -> str:
+def _precision_of(name: str):
+    """'f64' or 'f32' from a gate-kernel symbol (the split-product kernel is
+    f32 only), or None."""
+    if "<double" in name:
+        return "f64"
+    if "<float" in name or name.startswith("k_mfma_split"):
+        return "f32"
+    return None
+
+
+def kernel_trace(path: str, last: int | None, precision: str | None = None) -> str:
     with open(path, newline="") as f:
         rows = list(csv.DictReader(f))
     rows.sort(key=lambda r: int(r["Start_Timestamp"]))
@@ -54,7 +64,8 @@ def kernel_trace(path: str, last: int | None) -> str:
         name = _short(r["Kernel_Name"])
         ms = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6
         allk[name].append(ms)
-        if name.startswith(GATE_KERNELS):
+        if name.startswith(GATE_KERNELS) and (precision is None or
+                                              _precision_of(name) == precision):
             gate.append((name, ms))
     text = _table(allk, f"All dispatches ({len(rows)})")
     if last:
@@ -134,12 +145,14 @@ def main() -> None:
     ap.add_argument("--algorithmic", type=float, default=0.0)
     ap.add_argument("--kernels", nargs="+", default=None,
                     help="kernel-name prefixes for traffic mode (default: the gate kernels)")
+    ap.add_argument("--precision", choices=["f64", "f32"], default=None,
+                    help="kernel-trace mode: --last counts only gate kernels of this precision")
     a = ap.parse_args()
     if a.traffic:
         print(traffic(a.csv, a.traffic, a.copy_bytes, a.last or 0, a.algorithmic,
                       tuple(a.kernels) if a.kernels else GATE_KERNELS))
     else:
-        print(counters(a.csv) if a.counters else kernel_trace(a.csv, a.last))
+        print(counters(a.csv) if a.counters else kernel_trace(a.csv, a.last, a.precision))
 
 
 if __name__ == "__main__":
