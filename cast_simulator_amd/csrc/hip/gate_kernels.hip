@@ -670,17 +670,22 @@ uint32_t dense_family(const GatePlan &p, uint32_t n) {
   const uint32_t n_low = count_low(p);
   uint32_t v = p.variant;
   if (auto_choice(p)) {
-    // Automatic choice, from 30-qubit measurements on MI355X (the per-gate
-    // autotuner refines it where it runs):
+    // Automatic choice, from measurements on MI355X (the per-gate autotuner
+    // refines it where it runs):
     //  * wave tiles stream full rows and win up to 3 low-order targets for
     //    k <= 4; beyond that the exchanged register groups cost more than
     //    the matrix-core kernel's 16-combo tiles;
-    //  * k = 5: wave tiles only without low targets at f32 (FMA bound
-    //    otherwise) and with at most one at f64;
+    //  * k = 5: the matrix cores (f32: bf16 split products) for any
+    //    targets: on the config-3 circuit's k = 5 gates without autotuning,
+    //    wave tiles ran 4979 GB/s at 33 qubits f64 against 6089 on the
+    //    matrix cores, and 4849 at 34 qubits f32 against 5214; on the
+    //    30-qubit headline circuit the autotuner keeps the matrix cores for
+    //    most k = 5 gates (19 of 29 at f64, 21 of 29 at f32);
     //  * k = 6: only the matrix-core kernel keeps its operands out of
     //    scratch memory.
-    const uint32_t tile_low = p.k <= 4 ? 3 : (p.precision == 0 ? 0 : 1);
-    if (tile_ok && n_low <= tile_low)
+    if (p.k >= 5 && mfma_ok)
+      v = 4;
+    else if (tile_ok && n_low <= 3)
       v = 3;
     else if (mfma_ok)
       v = 4;
