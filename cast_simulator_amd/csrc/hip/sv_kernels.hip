@@ -454,12 +454,19 @@ constexpr int kMaxDevices = 64;
 std::atomic<double *> g_scratch_ptr[kMaxDevices];
 std::mutex g_scratch_mu;
 
-// Device address of `scratch` on the current device, looked up once per
-// device with that device current (the host-side name is not a device
-// address and must never be passed to a kernel).
-double *device_scratch() {
-  int dev = 0;
-  check(hipGetDevice(&dev), "hipGetDevice");
+// Device address of `scratch` on the device `stream` belongs to (the current
+// device for the null stream), looked up once per device with that device
+// current (the host-side name is not a device address and must never be
+// passed to a kernel).
+double *device_scratch(hipStream_t stream) {
+  int cur = 0;
+  check(hipGetDevice(&cur), "hipGetDevice");
+  int dev = cur;
+  if (stream) {
+    hipDevice_t sd = 0;
+    check(hipStreamGetDevice(stream, &sd), "hipStreamGetDevice");
+    dev = int(sd);
+  }
   if (dev < 0 || dev >= kMaxDevices)
     throw Error("device ordinal " + std::to_string(dev) + " out of range");
   double *p = g_scratch_ptr[dev].load(std::memory_order_acquire);
@@ -468,8 +475,13 @@ double *device_scratch() {
   std::lock_guard<std::mutex> lock(g_scratch_mu);
   p = g_scratch_ptr[dev].load(std::memory_order_relaxed);
   if (!p) {
+    if (dev != cur)
+      check(hipSetDevice(dev), "hipSetDevice");
     void *addr = nullptr;
-    check(hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    const hipError_t e = hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch));
+    if (dev != cur)
+      check(hipSetDevice(cur), "hipSetDevice");
+    check(e, "hipGetSymbolAddress(scratch)");
     if (!addr)
       throw Error("hipGetSymbolAddress(scratch) returned a null address");
     p = static_cast<double *>(addr);
@@ -489,7 +501,7 @@ void launch_observe_noalloc(const void *sv, uint32_t n_qubits, uint8_t precision
                 " qubit pairs, the size of the device scratch; got " + std::to_string(n_pairs));
   static_assert(sizeof(scratch) >= sizeof(double) * kMaxRedGrid * (1 + kMaxPairs),
                 "scratch must hold the largest launch_observe");
-  launch_observe(sv, n_qubits, precision, pairs, n_pairs, out, device_scratch(), stream);
+  launch_observe(sv, n_qubits, precision, pairs, n_pairs, out, device_scratch(stream), stream);
 }
 
 void launch_observe(const void *sv, uint32_t n_qubits, uint8_t precision, const uint32_t *pairs,
