@@ -67,7 +67,11 @@ struct GateGeom {
   uint64_t combos;
   uint32_t q[6];
   uint32_t k;
-  uint32_t pad;
+  // Matrix-core kernels: 0 = waves visit tiles in launch order; 1 = the
+  // grid (a multiple of 8 workgroups) is renumbered so that the workgroups
+  // dispatched to one XCD (every 8th) cover a contiguous 1/8 of each
+  // grid-stride step.
+  uint32_t xcd;
 };
 
 // Plan built by cast_hip_gate_plan_create.

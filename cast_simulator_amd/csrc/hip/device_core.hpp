@@ -9,6 +9,18 @@
 namespace cast_hip {
 namespace dev {
 
+// Index of the calling wave in a 1-D grid of workgroups.  With `xcd` set
+// (grid a multiple of 8) workgroup b is renumbered (b mod 8) * (grid / 8) +
+// b / 8: workgroups are dispatched to the 8 XCDs round-robin, so each XCD
+// then owns a contiguous range of wave indices.  A bijection on the waves.
+__device__ __forceinline__ uint64_t grid_wave(uint32_t xcd) {
+  uint64_t b = blockIdx.x;
+  if (xcd)
+    b = (b & 7u) * (gridDim.x >> 3) + (b >> 3);
+  return (b * blockDim.x + threadIdx.x) >> 6;
+}
+
+
 template <typename T> struct V2;
 template <> struct V2<float> {
   using type = float2;

@@ -221,7 +221,7 @@ __global__ void __launch_bounds__(256) k_mfma(T *__restrict__ sv, const T *__res
       dout[rb][i] = local_delta<KL>(16 * rb + (sizeof(T) == 8 ? j + 4 * i : 4 * j + i), g.q);
 
   const uint64_t n_waves = (uint64_t(gridDim.x) * blockDim.x) >> 6;
-  for (uint64_t t = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6; t < n_tiles;
+  for (uint64_t t = dev::grid_wave(g.xcd); t < n_tiles;
        t += n_waves) {
     const uint64_t base = combo_base<KL>(t * 16 + col, g.q);
     T br[KS], bi[KS], bs[KS];
@@ -283,7 +283,7 @@ __global__ void __launch_bounds__(kMfma6Block) k_mfma6(T *__restrict__ sv, const
   for (int s = 0; s < KS; ++s)
     din[s] = local_delta<KL>(4 * s + j, g.q);
   const uint64_t n_waves = (uint64_t(gridDim.x) * blockDim.x) >> 6;
-  for (uint64_t t = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6; t < n_tiles;
+  for (uint64_t t = dev::grid_wave(g.xcd); t < n_tiles;
        t += n_waves) {
     const uint64_t base = combo_base<KL>(t * 16 + col, g.q);
     T br[KS], bi[KS], bs[KS];
@@ -350,7 +350,7 @@ __global__ void __launch_bounds__(kSplitBlock) k_mfma_split(float *__restrict__ 
   float2 *__restrict__ amp = reinterpret_cast<float2 *>(sv);
   const dev::SplitLane<KL> sl(g.q);
   const uint64_t n_waves = (uint64_t(gridDim.x) * blockDim.x) >> 6;
-  for (uint64_t t = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6; t < n_tiles;
+  for (uint64_t t = dev::grid_wave(g.xcd); t < n_tiles;
        t += n_waves)
     dev::split_tile<KL>(amp, amp, a_lds, g.q, sl, t);
 }
@@ -448,8 +448,26 @@ bool use_split(const GatePlan &p);
 template <int KL>
 void launch_mfma_split(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &g);
 
+// Tile order of a matrix-core launch: XCD-contiguous (GateGeom::xcd) when
+// the plan asks for the XCD order (CAST_HIP_FLAG_TILE_XCD) or, for
+// measurements, CAST_HIP_MFMA_XCD=1; the grid is then rounded down to a
+// multiple of 8 (the kernels' grid-stride loops cover every tile).
+unsigned mfma_order(const GatePlan &p, unsigned grid, GateGeom &g) {
+  static const bool env = [] {
+    const char *e = std::getenv("CAST_HIP_MFMA_XCD");
+    return e && std::string(e) == "1";
+  }();
+  g.xcd = 0;
+  if ((env || (p.flags & CAST_HIP_FLAG_TILE_XCD)) && grid >= 8) {
+    g.xcd = 1;
+    grid &= ~7u;
+  }
+  return grid;
+}
+
 template <typename T, int KL>
-void launch_mfma_t(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &g) {
+void launch_mfma_t(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &g0) {
+  GateGeom g = g0;
   if constexpr (sizeof(T) == 4 && KL == 5) {
     if (use_split(p))
       return launch_mfma_split<5>(p, s, sv, g);
@@ -475,7 +493,7 @@ void launch_mfma_t(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &g
     cap = n_cu * per_cu_env;
   else if (sizeof(T) == 8)
     cap = std::min(n_cu * 128, std::max(cap, tiles / 64));
-  const unsigned grid = unsigned(std::max<uint64_t>(1, std::min(need, cap)));
+  const unsigned grid = mfma_order(p, unsigned(std::max<uint64_t>(1, std::min(need, cap))), g);
   hipLaunchKernelGGL((k_mfma<T, KL>), dim3(grid), dim3(kBlock), 0, s, static_cast<T *>(sv),
                      static_cast<const T *>(p.d_mat3), g, tiles);
   check(hipGetLastError(), "hipLaunchKernelGGL(k_mfma)");
@@ -511,7 +529,8 @@ bool use_split(const GatePlan &p) {
 }
 
 template <int KL>
-void launch_mfma_split(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &g) {
+void launch_mfma_split(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &g0) {
+  GateGeom g = g0;
   const uint64_t tiles = g.combos / 16;
   const uint64_t need = (tiles + kSplitBlock / 64 - 1) / (kSplitBlock / 64);
   // Grid: at most 48 workgroups per CU (k = 6 keeps 48 KiB of LDS per
@@ -527,14 +546,15 @@ void launch_mfma_split(const GatePlan &p, hipStream_t s, void *sv, const GateGeo
   }();
   const uint64_t n_cu = uint64_t(std::max(1, device_props().n_cu));
   const uint64_t cap = n_cu * (per_cu_env ? per_cu_env : 48);
-  const unsigned grid = unsigned(std::max<uint64_t>(1, std::min(need, cap)));
+  const unsigned grid = mfma_order(p, unsigned(std::max<uint64_t>(1, std::min(need, cap))), g);
   hipLaunchKernelGGL((k_mfma_split<KL>), dim3(grid), dim3(kSplitBlock), 0, s,
                      static_cast<float *>(sv), static_cast<const float *>(p.d_mat3), g, tiles);
   check(hipGetLastError(), "hipLaunchKernelGGL(k_mfma_split)");
 }
 
 template <typename T>
-void launch_mfma6_t(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &g) {
+void launch_mfma6_t(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &g0) {
+  GateGeom g = g0;
   if constexpr (sizeof(T) == 4) {
     if (use_split(p))
       return launch_mfma_split<6>(p, s, sv, g);
@@ -556,7 +576,7 @@ void launch_mfma6_t(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &
   const uint64_t n_cu = uint64_t(std::max(1, device_props().n_cu));
   const uint64_t cap = per_cu_env ? n_cu * per_cu_env
                                   : std::min(n_cu * 32, std::max(n_cu * 2, tiles / 2048));
-  const unsigned grid = unsigned(std::max<uint64_t>(1, std::min(need, cap)));
+  const unsigned grid = mfma_order(p, unsigned(std::max<uint64_t>(1, std::min(need, cap))), g);
   hipLaunchKernelGGL((k_mfma6<T>), dim3(grid), dim3(kMfma6Block), 0, s, static_cast<T *>(sv),
                      static_cast<const T *>(p.d_mat3), g, tiles);
   check(hipGetLastError(), "hipLaunchKernelGGL(k_mfma6)");

@@ -96,7 +96,8 @@ struct PlanInfo {
 
 // Per-gate kernel autotuning: for a dense gate on a statevector of at least
 // `min_qubits` qubits, every applicable kernel (wave tile in three orders and
-// two product forms, matrix core, direct) is timed on the real gate, and the
+// two product forms, matrix core in two orders and, at f32 k >= 5, two
+// product forms, direct) is timed on the real gate, and the
 // fastest (by minimum) is used from then on.  Keyed by (kernel id, n).
 // Successive halving: every candidate is timed once; candidates slower than
 // kTuneMargin x the best so far are dropped; the rest are timed until each
@@ -115,6 +116,9 @@ constexpr uint32_t kTileGaussXcd = 9;
 // ... and of the matrix-core kernel with f32 products for f32 k = 5, 6
 // (CAST_HIP_FLAG_MFMA_F32), whose default there is the bf16 split products.
 constexpr uint32_t kMfmaF32 = 10;
+// ... and of the matrix-core kernel (default product form) in XCD-blocked
+// order (CAST_HIP_FLAG_TILE_XCD).
+constexpr uint32_t kMfmaXcd = 11;
 
 struct TuneEntry {
   KernelId id = 0;

@@ -79,7 +79,9 @@ typedef struct cast_hip_kernel_gen_spec_t {
 #define CAST_HIP_FLAG_TILE_GAUSS 2u
 /* Visit wave tiles so that the blocks dispatched to each of the 8 XCDs
  * (block b runs on XCD b mod 8) cover one contiguous eighth of the tiles,
- * instead of every XCD interleaving with the others.  Faster for some
+ * instead of every XCD interleaving with the others.  The matrix-core
+ * kernels (grid-stride loops) take the same flag: each XCD's workgroups then
+ * cover a contiguous eighth of every grid-stride step.  Faster for some
  * target sets and slower for others; the host autotuner tries it. */
 #define CAST_HIP_FLAG_TILE_XCD 4u
 /* Dense k = 6 at F32 on the matrix cores: through bf16 split products
