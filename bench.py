@@ -35,17 +35,17 @@ SEED = 20260101
 KERNEL = {1: "k_dense", 2: "k_monomial", 3: "k_tile", 4: "k_mfma", 5: "k_tile(swizzled)",
           6: "k_tile(3 products)", 7: "k_tile(3 products, swizzled)", 8: "k_tile(xcd order)",
           9: "k_tile(3 products, xcd order)", 10: "k_mfma(f32 products)",
-          11: "k_mfma(xcd order)"}
+          11: "k_mfma(xcd order)", 12: "k_mfma(swizzled)"}
 
 
 def mfma_symbol(f: int, ctype: str, k: int) -> str:
     """Profiler symbol of a matrix-core or monomial launch: f32 k = 5, 6 run
     the bf16 split-product kernel unless the f32-product kernel (code 10) was
-    chosen; code 11 is the default kernel in XCD-blocked tile order (same
-    symbol)."""
-    if f in (4, 11) and ctype == "float" and k >= 5:
+    chosen; codes 11 and 12 are the default kernel in XCD-blocked and in
+    swizzled tile order (same symbol)."""
+    if f in (4, 11, 12) and ctype == "float" and k >= 5:
         return f"k_mfma_split<{k}>"
-    if f in (4, 10, 11):
+    if f in (4, 10, 11, 12):
         return f"k_mfma6<{ctype}>" if k == 6 else f"k_mfma<{ctype}, {k}>"
     return f"{KERNEL[f]}<{ctype}, {k}>"
 
@@ -304,7 +304,7 @@ def run_dense_large(cs, n64: int = 33, n32: int = 34, depth: int = 12):
             k = info[kid].k
             dense = info[kid].scalar_nnz == 2 * (1 << k) ** 2
             f = fam[kid]
-            kern = mfma_symbol(f, ctype, k) if f in (4, 10, 11) else f"{KERNEL[f]}<{prec}, k={k}>"
+            kern = mfma_symbol(f, ctype, k) if f in (4, 10, 11, 12) else f"{KERNEL[f]}<{prec}, k={k}>"
             groups[(kern, k, dense)].append(t)
         rows = []
         for (kern, k, dense), ts in sorted(groups.items(), key=lambda kv: -sum(kv[1])):

@@ -72,6 +72,10 @@ struct GateGeom {
   // dispatched to one XCD (every 8th) cover a contiguous 1/8 of each
   // grid-stride step.
   uint32_t xcd;
+  // Matrix-core kernels: 0, or the number of low tile-index bits visited in
+  // bit-reversed order (n_tiles is a power of two >= 2^swz).
+  uint32_t swz;
+  uint32_t pad;
 };
 
 // Plan built by cast_hip_gate_plan_create.

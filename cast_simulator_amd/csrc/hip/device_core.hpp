@@ -20,6 +20,15 @@ __device__ __forceinline__ uint64_t grid_wave(uint32_t xcd) {
   return (b * blockDim.x + threadIdx.x) >> 6;
 }
 
+// Tile visited at loop index t: the low `swz` bits of t bit-reversed (0 =
+// launch order).  A bijection on [0, 2^m) for every m >= swz.
+__device__ __forceinline__ uint64_t swizzled_tile(uint64_t t, uint32_t swz) {
+  if (!swz)
+    return t;
+  const uint64_t lo = t & ((uint64_t(1) << swz) - 1);
+  return (t ^ lo) | (__brevll(lo) >> (64 - swz));
+}
+
 
 template <typename T> struct V2;
 template <> struct V2<float> {

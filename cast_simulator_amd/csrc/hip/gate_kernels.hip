@@ -223,7 +223,7 @@ __global__ void __launch_bounds__(256) k_mfma(T *__restrict__ sv, const T *__res
   const uint64_t n_waves = (uint64_t(gridDim.x) * blockDim.x) >> 6;
   for (uint64_t t = dev::grid_wave(g.xcd); t < n_tiles;
        t += n_waves) {
-    const uint64_t base = combo_base<KL>(t * 16 + col, g.q);
+    const uint64_t base = combo_base<KL>(dev::swizzled_tile(t, g.swz) * 16 + col, g.q);
     T br[KS], bi[KS], bs[KS];
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
@@ -285,7 +285,7 @@ __global__ void __launch_bounds__(kMfma6Block) k_mfma6(T *__restrict__ sv, const
   const uint64_t n_waves = (uint64_t(gridDim.x) * blockDim.x) >> 6;
   for (uint64_t t = dev::grid_wave(g.xcd); t < n_tiles;
        t += n_waves) {
-    const uint64_t base = combo_base<KL>(t * 16 + col, g.q);
+    const uint64_t base = combo_base<KL>(dev::swizzled_tile(t, g.swz) * 16 + col, g.q);
     T br[KS], bi[KS], bs[KS];
 #pragma unroll
     for (int s = 0; s < KS; ++s) {
@@ -352,7 +352,7 @@ __global__ void __launch_bounds__(kSplitBlock) k_mfma_split(float *__restrict__ 
   const uint64_t n_waves = (uint64_t(gridDim.x) * blockDim.x) >> 6;
   for (uint64_t t = dev::grid_wave(g.xcd); t < n_tiles;
        t += n_waves)
-    dev::split_tile<KL>(amp, amp, a_lds, g.q, sl, t);
+    dev::split_tile<KL>(amp, amp, a_lds, g.q, sl, dev::swizzled_tile(t, g.swz));
 }
 
 // ── Batched dense: item = blockIdx.z ────────────────────────────────────────
@@ -448,19 +448,28 @@ bool use_split(const GatePlan &p);
 template <int KL>
 void launch_mfma_split(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &g);
 
-// Tile order of a matrix-core launch: XCD-contiguous (GateGeom::xcd) when
-// the plan asks for the XCD order (CAST_HIP_FLAG_TILE_XCD) or, for
-// measurements, CAST_HIP_MFMA_XCD=1; the grid is then rounded down to a
-// multiple of 8 (the kernels' grid-stride loops cover every tile).
-unsigned mfma_order(const GatePlan &p, unsigned grid, GateGeom &g) {
+// Tile order of a matrix-core launch over n_tiles (a power of two) tiles:
+// XCD-contiguous (GateGeom::xcd) when the plan asks for the XCD order
+// (CAST_HIP_FLAG_TILE_XCD) or, for measurements, CAST_HIP_MFMA_XCD=1, with
+// the grid rounded down to a multiple of 8 (the kernels' grid-stride loops
+// cover every tile); the low 10 tile-index bits bit-reversed
+// (GateGeom::swz) when it asks for CAST_HIP_FLAG_TILE_SWIZZLE.
+unsigned mfma_order(const GatePlan &p, unsigned grid, uint64_t n_tiles, GateGeom &g) {
   static const bool env = [] {
     const char *e = std::getenv("CAST_HIP_MFMA_XCD");
     return e && std::string(e) == "1";
   }();
   g.xcd = 0;
+  g.swz = 0;
   if ((env || (p.flags & CAST_HIP_FLAG_TILE_XCD)) && grid >= 8) {
     g.xcd = 1;
     grid &= ~7u;
+  }
+  if ((p.flags & CAST_HIP_FLAG_TILE_SWIZZLE) && n_tiles >= 2) {
+    uint32_t m = 0;
+    while ((uint64_t(2) << m) <= n_tiles)
+      ++m; // n_tiles = 2^m
+    g.swz = std::min<uint32_t>(10, m);
   }
   return grid;
 }
@@ -493,7 +502,7 @@ void launch_mfma_t(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &g
     cap = n_cu * per_cu_env;
   else if (sizeof(T) == 8)
     cap = std::min(n_cu * 128, std::max(cap, tiles / 64));
-  const unsigned grid = mfma_order(p, unsigned(std::max<uint64_t>(1, std::min(need, cap))), g);
+  const unsigned grid = mfma_order(p, unsigned(std::max<uint64_t>(1, std::min(need, cap))), tiles, g);
   hipLaunchKernelGGL((k_mfma<T, KL>), dim3(grid), dim3(kBlock), 0, s, static_cast<T *>(sv),
                      static_cast<const T *>(p.d_mat3), g, tiles);
   check(hipGetLastError(), "hipLaunchKernelGGL(k_mfma)");
@@ -546,7 +555,7 @@ void launch_mfma_split(const GatePlan &p, hipStream_t s, void *sv, const GateGeo
   }();
   const uint64_t n_cu = uint64_t(std::max(1, device_props().n_cu));
   const uint64_t cap = n_cu * (per_cu_env ? per_cu_env : 48);
-  const unsigned grid = mfma_order(p, unsigned(std::max<uint64_t>(1, std::min(need, cap))), g);
+  const unsigned grid = mfma_order(p, unsigned(std::max<uint64_t>(1, std::min(need, cap))), tiles, g);
   hipLaunchKernelGGL((k_mfma_split<KL>), dim3(grid), dim3(kSplitBlock), 0, s,
                      static_cast<float *>(sv), static_cast<const float *>(p.d_mat3), g, tiles);
   check(hipGetLastError(), "hipLaunchKernelGGL(k_mfma_split)");
@@ -576,7 +585,7 @@ void launch_mfma6_t(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &
   const uint64_t n_cu = uint64_t(std::max(1, device_props().n_cu));
   const uint64_t cap = per_cu_env ? n_cu * per_cu_env
                                   : std::min(n_cu * 32, std::max(n_cu * 2, tiles / 2048));
-  const unsigned grid = mfma_order(p, unsigned(std::max<uint64_t>(1, std::min(need, cap))), g);
+  const unsigned grid = mfma_order(p, unsigned(std::max<uint64_t>(1, std::min(need, cap))), tiles, g);
   hipLaunchKernelGGL((k_mfma6<T>), dim3(grid), dim3(kMfma6Block), 0, s, static_cast<T *>(sv),
                      static_cast<const T *>(p.d_mat3), g, tiles);
   check(hipGetLastError(), "hipLaunchKernelGGL(k_mfma6)");

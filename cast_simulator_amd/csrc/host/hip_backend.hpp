@@ -96,7 +96,7 @@ struct PlanInfo {
 
 // Per-gate kernel autotuning: for a dense gate on a statevector of at least
 // `min_qubits` qubits, every applicable kernel (wave tile in three orders and
-// two product forms, matrix core in two orders and, at f32 k >= 5, two
+// two product forms, matrix core in three orders and, at f32 k >= 5, two
 // product forms, direct) is timed on the real gate, and the
 // fastest (by minimum) is used from then on.  Keyed by (kernel id, n).
 // Successive halving: every candidate is timed once; candidates slower than
@@ -117,8 +117,10 @@ constexpr uint32_t kTileGaussXcd = 9;
 // (CAST_HIP_FLAG_MFMA_F32), whose default there is the bf16 split products.
 constexpr uint32_t kMfmaF32 = 10;
 // ... and of the matrix-core kernel (default product form) in XCD-blocked
-// order (CAST_HIP_FLAG_TILE_XCD).
+// order (CAST_HIP_FLAG_TILE_XCD) and in swizzled order
+// (CAST_HIP_FLAG_TILE_SWIZZLE).
 constexpr uint32_t kMfmaXcd = 11;
+constexpr uint32_t kMfmaSwizzled = 12;
 
 struct TuneEntry {
   KernelId id = 0;

@@ -69,9 +69,10 @@ typedef struct cast_hip_kernel_gen_spec_t {
   uint32_t flags; /* CAST_HIP_FLAG_* */
 } cast_hip_kernel_gen_spec_t;
 
-/* Visit wave tiles in bit-reversed order of their low index bits.  Which
- * order streams faster depends on how the gate's target bits fall on the
- * HBM channel/bank mapping; the host autotuner tries both. */
+/* Visit wave tiles (and matrix-core tiles: the low 10 index bits) in
+ * bit-reversed order of their low index bits.  Which order streams faster
+ * depends on how the gate's target bits fall on the HBM channel/bank
+ * mapping; the host autotuner tries both. */
 #define CAST_HIP_FLAG_TILE_SWIZZLE 1u
 /* Wave tiles of k = 4, 5 in the three-product form (3 K instead of 4 K FMAs
  * per output: Re = Mr vr - Mi vi, Im = (Mr + Mi)(vr + vi) - Mr vr - Mi vi),
