@@ -820,8 +820,8 @@ def main() -> int:
                          "(a known-size transfer that calibrates profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); the "
-              "multi-GPU design is described in DESIGN.md and is not run here.", file=sys.stderr)
+        print(f"bench.py: measures one GPU only (got --gpus {args.gpus}); sharding a "
+              "statevector over several GPUs is not implemented.", file=sys.stderr)
         return 2
 
     import cast_simulator_amd as cs

@@ -12,5 +12,5 @@ if str(TESTS) not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line(
-        "markers", "gpu: needs an MI355X (gfx950) device; run with -m gpu on the GPU box"
+        "markers", "gpu: needs an MI355X (gfx950) device; run with -m gpu on a machine that has one"
     )
