@@ -544,19 +544,28 @@ void launch_mfma_split(const GatePlan &p, hipStream_t s, void *sv, const GateGeo
   GateGeom g = g0;
   const uint64_t tiles = g.combos / 16;
   const uint64_t need = (tiles + kSplitBlock / 64 - 1) / (kSplitBlock / 64);
-  // Grid: at most 48 workgroups per CU (k = 6 keeps 48 KiB of LDS per
-  // workgroup, 3 resident per CU; each stages the split matrices once).
-  // Measured at 30 qubits on the config-3 circuit's gates (19 k = 5, 9 k = 6):
-  // caps of 3, 8, 16 and 48 per CU gave 4799 / 4766 / 4822 / 4899 GB/s at
-  // k = 5 and 4541 / 4544 / 4588 / 4633 at k = 6.
+  // Grid (each workgroup stages the split matrices once):
+  //  * k = 6: at most 48 workgroups per CU (48 KiB of LDS per workgroup, 3
+  //    resident per CU).  At 30 qubits on the config-3 circuit's 9 k = 6
+  //    gates, caps of 3, 8, 16, 48, 128, 256, 512 per CU gave 4541 / 4544 /
+  //    4588 / 4633 / 4446 / 4316 / 4020 GB/s.
+  //  * k = 5: about 4 tiles per wave, between 48 and 512 workgroups per CU.
+  //    Caps of 48, 128, 256, 512, 1024 and 2048 (one tile per wave) per CU
+  //    gave 4828 / 4886 / 4953 / 5009-5027 / 5057 / 4830 GB/s on the 29 k = 5
+  //    gates of the 30-qubit headline circuit at f32, and 5313-5489 at 512
+  //    against 5346 at 48 on the 34-qubit config-3 gates.
   // CAST_HIP_SPLIT_BLOCKS_PER_CU fixes the cap for measurements.
   static const unsigned per_cu_env = [] {
     const char *e = std::getenv("CAST_HIP_SPLIT_BLOCKS_PER_CU");
     long v = e && *e ? std::strtol(e, nullptr, 10) : 0;
-    return unsigned(v > 0 && v <= 1024 ? v : 0);
+    return unsigned(v > 0 && v <= 4096 ? v : 0);
   }();
   const uint64_t n_cu = uint64_t(std::max(1, device_props().n_cu));
-  const uint64_t cap = n_cu * (per_cu_env ? per_cu_env : 48);
+  uint64_t cap = n_cu * 48;
+  if (per_cu_env)
+    cap = n_cu * per_cu_env;
+  else if (KL == 5)
+    cap = std::min(n_cu * 512, std::max(cap, tiles / 16));
   const unsigned grid = mfma_order(p, unsigned(std::max<uint64_t>(1, std::min(need, cap))), tiles, g);
   hipLaunchKernelGGL((k_mfma_split<KL>), dim3(grid), dim3(kSplitBlock), 0, s,
                      static_cast<float *>(sv), static_cast<const float *>(p.d_mat3), g, tiles);
