@@ -485,12 +485,13 @@ void launch_mfma_t(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &g
   }
   // Grid: the three real matrices are loaded into registers once per wave,
   // and long grid-stride loops stream worse than full grids.  At f64 the
-  // grid gives each wave about 16 tiles, between 8 and 128 workgroups per
+  // grid gives each wave about 8 tiles, between 8 and 256 workgroups per
   // CU; at f32, 8 per CU.  Measured on the 29 k = 5 gates of the 30-qubit
   // f64 headline circuit: 4898 GB/s aggregate at 8 per CU (256 tiles per
-  // wave), 5118 at 32, 5232 at 128 (16 tiles per wave), 5049 at 512; the
-  // serving workload (12-28 qubits, f32 and f64): 79.3 circuits/s with 8
-  // per CU, 72.3 with 128.
+  // wave), 5118 at 32, 5255 at 64, 5232-5271 at 128, 5312 at 192, 5372 at
+  // 256 (8 tiles per wave), 5286 at 384, 5049 at 512; the serving workload
+  // (12-28 qubits, f32 and f64): 79.3 circuits/s with 8 per CU, 72.3 with
+  // 128.
   static const unsigned per_cu_env = [] {
     const char *e = std::getenv("CAST_HIP_MFMA_BLOCKS_PER_CU");
     long v = e && *e ? std::strtol(e, nullptr, 10) : 0;
@@ -503,7 +504,7 @@ void launch_mfma_t(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &g
   if (per_cu_env)
     cap = n_cu * per_cu_env;
   else if (sizeof(T) == 8)
-    cap = std::min(n_cu * 128, std::max(cap, tiles / 64));
+    cap = std::min(n_cu * 256, std::max(cap, tiles / 32));
   const unsigned grid = mfma_order(p, unsigned(std::max<uint64_t>(1, std::min(need, cap))), tiles, g);
   hipLaunchKernelGGL((k_mfma<T, KL>), dim3(grid), dim3(kBlock), 0, s, static_cast<T *>(sv),
                      static_cast<const T *>(p.d_mat3), g, tiles);
