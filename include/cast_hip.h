@@ -272,7 +272,8 @@ int cast_hip_observe_async(uint64_t sv_dptr, uint32_t sv_n_qubits, uint8_t preci
 
 /* cast_hip_observe_async without the scratch argument: the reduction
  * partials live in a static device array of the library (528 KiB per GPU,
- * held until the process ends), so the call allocates nothing (no hipMalloc,
+ * held until the process ends; the copy on the device of `stream`, or of
+ * the current device for the null stream), so the call allocates nothing (no hipMalloc,
  * no pinned memory) and only enqueues two kernels on `stream`.  The caller
  * passes the statevector and the device output (1 + n_pairs doubles);
  * n_pairs above 32 is rejected.  Every call on one GPU shares that GPU's
