@@ -389,6 +389,7 @@ __global__ void __launch_bounds__(256) k_dense_batch(BatchArgs a) {
 constexpr unsigned kBlock = 256;
 
 bool tile_gauss(const GatePlan &p); // defined with the family rule below
+bool auto_choice(const GatePlan &p);
 
 unsigned grid_for(uint64_t combos) {
   static const unsigned cap = [] {
@@ -452,18 +453,21 @@ void launch_mfma_split(const GatePlan &p, hipStream_t s, void *sv, const GateGeo
 
 // Tile order of a matrix-core launch over n_tiles (a power of two) tiles:
 // XCD-contiguous (GateGeom::xcd) when the plan asks for the XCD order
-// (CAST_HIP_FLAG_TILE_XCD) or, for measurements, CAST_HIP_MFMA_XCD=1, with
-// the grid rounded down to a multiple of 8 (the kernels' grid-stride loops
-// cover every tile); the low 10 tile-index bits bit-reversed
-// (GateGeom::swz) when it asks for CAST_HIP_FLAG_TILE_SWIZZLE.
-unsigned mfma_order(const GatePlan &p, unsigned grid, uint64_t n_tiles, GateGeom &g) {
+// (CAST_HIP_FLAG_TILE_XCD), when `xcd_auto` is set and the family was chosen
+// automatically, or, for measurements, with CAST_HIP_MFMA_XCD=1; the grid is
+// then rounded down to a multiple of 8 (the kernels' grid-stride loops
+// cover every tile).  The low 10 tile-index bits bit-reversed (GateGeom::swz)
+// when the plan asks for CAST_HIP_FLAG_TILE_SWIZZLE.
+unsigned mfma_order(const GatePlan &p, unsigned grid, uint64_t n_tiles, GateGeom &g,
+                    bool xcd_auto = false) {
   static const bool env = [] {
     const char *e = std::getenv("CAST_HIP_MFMA_XCD");
     return e && std::string(e) == "1";
   }();
   g.xcd = 0;
   g.swz = 0;
-  if ((env || (p.flags & CAST_HIP_FLAG_TILE_XCD)) && grid >= 8) {
+  const bool xcd = env || (p.flags & CAST_HIP_FLAG_TILE_XCD) || (xcd_auto && auto_choice(p));
+  if (xcd && grid >= 8) {
     g.xcd = 1;
     grid &= ~7u;
   }
@@ -597,7 +601,12 @@ void launch_mfma6_t(const GatePlan &p, hipStream_t s, void *sv, const GateGeom &
   const uint64_t n_cu = uint64_t(std::max(1, device_props().n_cu));
   const uint64_t cap = per_cu_env ? n_cu * per_cu_env
                                   : std::min(n_cu * 32, std::max(n_cu * 2, tiles / 2048));
-  const unsigned grid = mfma_order(p, unsigned(std::max<uint64_t>(1, std::min(need, cap))), tiles, g);
+  // f64 launches chosen automatically visit tiles in XCD order: on the
+  // config-3 circuit's k = 6 gates 4319 -> 4687 GB/s at 28 qubits, 4633 ->
+  // 4920 at 30, unchanged at 26 and 33 (matrix-core bound there); f32 (the
+  // f32-product form) keeps launch order.
+  const unsigned grid = mfma_order(p, unsigned(std::max<uint64_t>(1, std::min(need, cap))), tiles,
+                                   g, sizeof(T) == 8);
   hipLaunchKernelGGL((k_mfma6<T>), dim3(grid), dim3(kMfma6Block), 0, s, static_cast<T *>(sv),
                      static_cast<const T *>(p.d_mat3), g, tiles);
   check(hipGetLastError(), "hipLaunchKernelGGL(k_mfma6)");
