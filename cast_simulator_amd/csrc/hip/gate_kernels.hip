@@ -673,8 +673,15 @@ TileGeom tile_geom(const GatePlan &p, uint32_t n, uint32_t *n_low) {
   const uint32_t L = t.n_low;
   if (p.flags & CAST_HIP_FLAG_TILE_SWIZZLE)
     tg.swz = std::min<uint32_t>(10, n - 6 - p.k);
-  // blocks = n_tiles / 4 (4 waves per block, one tile each), a power of two
-  if ((p.flags & CAST_HIP_FLAG_TILE_XCD) && tg.n_tiles >= 32)
+  // blocks = n_tiles / 4 (4 waves per block, one tile each), a power of two.
+  // Automatically chosen tiles (no order flag) use the XCD order too: on the
+  // config-3 circuit's gates forced to wave tiles it streamed 5504 -> 6360
+  // GB/s (k = 3) and 5524 -> 6316 (k = 4) at 33 qubits f64, 5507 -> 6474
+  // (k = 3) at 34 qubits f32, and +1.4 to +5.7 % on the 30-qubit headline
+  // circuit's k = 3, 4 gates; the per-gate autotuner still tries all three.
+  const bool xcd = (p.flags & CAST_HIP_FLAG_TILE_XCD) ||
+                   (auto_choice(p) && !(p.flags & CAST_HIP_FLAG_TILE_SWIZZLE));
+  if (xcd && tg.n_tiles >= 32)
     tg.xcd_bits = n - 6 - p.k - 2;
   *n_low = L;
   return tg;
