@@ -304,7 +304,13 @@ def run_dense_large(cs, n64: int = 33, n32: int = 34, depth: int = 12):
             k = info[kid].k
             dense = info[kid].scalar_nnz == 2 * (1 << k) ** 2
             f = fam[kid]
-            kern = mfma_symbol(f, ctype, k) if f in (4, 10, 11, 12) else f"{KERNEL[f]}<{prec}, k={k}>"
+            if f in (4, 10, 11, 12):
+                kern = mfma_symbol(f, ctype, k)
+            elif f == 3 and (k == 4 or (k == 5 and prec == "f32")):
+                # automatic wave tiles take the three-product form here
+                kern = f"k_tile3<{prec}, k={k}>"
+            else:
+                kern = f"{KERNEL[f]}<{prec}, k={k}>"
             groups[(kern, k, dense)].append(t)
         rows = []
         for (kern, k, dense), ts in sorted(groups.items(), key=lambda kv: -sum(kv[1])):
