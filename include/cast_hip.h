@@ -82,8 +82,11 @@ typedef struct cast_hip_kernel_gen_spec_t {
  * (block b runs on XCD b mod 8) cover one contiguous eighth of the tiles,
  * instead of every XCD interleaving with the others.  The matrix-core
  * kernels (grid-stride loops) take the same flag: each XCD's workgroups then
- * cover a contiguous eighth of every grid-stride step.  Faster for some
- * target sets and slower for others; the host autotuner tries it. */
+ * cover a contiguous eighth of every grid-stride step.  Faster for most
+ * target sets and slower for some; the host autotuner tries it.  With
+ * variant AUTO (no order flag) the library uses it for wave tiles and for
+ * F64 matrix-core launches; an explicitly requested variant without this
+ * flag keeps launch order. */
 #define CAST_HIP_FLAG_TILE_XCD 4u
 /* Dense k = 6 at F32 on the matrix cores: through bf16 split products
  * (every f32 operand split into three bf16 parts, six partial products per
