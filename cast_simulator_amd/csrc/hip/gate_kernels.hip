@@ -575,7 +575,12 @@ void launch_mfma_split(const GatePlan &p, hipStream_t s, void *sv, const GateGeo
     cap = n_cu * per_cu_env;
   else if (KL == 5)
     cap = std::min(n_cu * 512, std::max(cap, tiles / 16));
-  const unsigned grid = mfma_order(p, unsigned(std::max<uint64_t>(1, std::min(need, cap))), tiles, g);
+  // Automatically chosen launches on 32 qubits or more visit tiles in XCD
+  // order: on the config-3 circuit's gates at 34 qubits k = 5 5345 -> 5419
+  // GB/s, k = 6 4906 -> 4933; at 30 qubits it cost 1 % (k = 5).
+  const bool large = tiles >= (uint64_t(1) << (32 - KL - 4));
+  const unsigned grid = mfma_order(p, unsigned(std::max<uint64_t>(1, std::min(need, cap))), tiles,
+                                   g, large);
   hipLaunchKernelGGL((k_mfma_split<KL>), dim3(grid), dim3(kSplitBlock), 0, s,
                      static_cast<float *>(sv), static_cast<const float *>(p.d_mat3), g, tiles);
   check(hipGetLastError(), "hipLaunchKernelGGL(k_mfma_split)");

@@ -85,7 +85,7 @@ typedef struct cast_hip_kernel_gen_spec_t {
  * cover a contiguous eighth of every grid-stride step.  Faster for most
  * target sets and slower for some; the host autotuner tries it.  With
  * variant AUTO (no order flag) the library uses it for wave tiles and for
- * F64 matrix-core launches; an explicitly requested variant without this
+ * F64 matrix-core launches (F32 split products from 32 qubits); an explicitly requested variant without this
  * flag keeps launch order. */
 #define CAST_HIP_FLAG_TILE_XCD 4u
 /* Dense k = 6 at F32 on the matrix cores: through bf16 split products
