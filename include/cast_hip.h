@@ -84,9 +84,10 @@ typedef struct cast_hip_kernel_gen_spec_t {
  * kernels (grid-stride loops) take the same flag: each XCD's workgroups then
  * cover a contiguous eighth of every grid-stride step.  Faster for most
  * target sets and slower for some; the host autotuner tries it.  With
- * variant AUTO (no order flag) the library uses it for wave tiles and for
- * F64 matrix-core launches (F32 split products from 32 qubits); an explicitly requested variant without this
- * flag keeps launch order. */
+ * variant AUTO (no order flag) the library uses it for wave tiles, for F64
+ * matrix-core launches and for F32 split-product launches from 32 qubits
+ * on; an explicitly requested variant without this flag keeps launch
+ * order. */
 #define CAST_HIP_FLAG_TILE_XCD 4u
 /* Dense k = 6 at F32 on the matrix cores: through bf16 split products
  * (every f32 operand split into three bf16 parts, six partial products per
