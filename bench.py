@@ -35,7 +35,8 @@ SEED = 20260101
 KERNEL = {1: "k_dense", 2: "k_monomial", 3: "k_tile", 4: "k_mfma", 5: "k_tile(swizzled)",
           6: "k_tile(3 products)", 7: "k_tile(3 products, swizzled)", 8: "k_tile(xcd order)",
           9: "k_tile(3 products, xcd order)", 10: "k_mfma(f32 products)",
-          11: "k_mfma(xcd order)", 12: "k_mfma(swizzled)"}
+          11: "k_mfma(xcd order)", 12: "k_mfma(swizzled)", 13: "k_tile(xcd order, swizzled)",
+          14: "k_tile(3 products, xcd order, swizzled)"}
 
 
 def mfma_symbol(f: int, ctype: str, k: int) -> str:
@@ -105,9 +106,9 @@ def run_gpu(cs, n: int, precision: str, steps: int, warmup: int, max_size: int):
     def symbol(kid):
         """Kernel symbol as the profiler names it."""
         k, f = info[kid].k, fam[kid]
-        if f in (3, 5, 6, 7, 8, 9):
+        if f in (3, 5, 6, 7, 8, 9, 13, 14):
             low = sum(1 for q in info[kid].qubits if q < 6)
-            return (f"k_tile3<{ctype}, {k}, {low}>" if f in (6, 7, 9)
+            return (f"k_tile3<{ctype}, {k}, {low}>" if f in (6, 7, 9, 14)
                     else f"k_tile<{ctype}, {k}, {low}>")
         if f == 1:  # dense: the store mode is a template argument
             nts = precision == "f64" and min(info[kid].qubits) >= 3

@@ -167,6 +167,9 @@ HipKernelManager::TuneState &HipKernelManager::tune_state_locked(KernelId id, ui
       {CAST_HIP_VARIANT_TILE, CAST_HIP_FLAG_TILE_GAUSS | CAST_HIP_FLAG_TILE_SWIZZLE},
       {CAST_HIP_VARIANT_TILE, CAST_HIP_FLAG_TILE_XCD},
       {CAST_HIP_VARIANT_TILE, CAST_HIP_FLAG_TILE_GAUSS | CAST_HIP_FLAG_TILE_XCD},
+      {CAST_HIP_VARIANT_TILE, CAST_HIP_FLAG_TILE_XCD | CAST_HIP_FLAG_TILE_SWIZZLE},
+      {CAST_HIP_VARIANT_TILE,
+       CAST_HIP_FLAG_TILE_GAUSS | CAST_HIP_FLAG_TILE_XCD | CAST_HIP_FLAG_TILE_SWIZZLE},
       {CAST_HIP_VARIANT_MFMA, 0u},
       {CAST_HIP_VARIANT_MFMA, CAST_HIP_FLAG_MFMA_F32},
       {CAST_HIP_VARIANT_MFMA, CAST_HIP_FLAG_TILE_XCD},
@@ -192,7 +195,8 @@ HipKernelManager::TuneState &HipKernelManager::tune_state_locked(KernelId id, ui
     const bool f32_products = c.second & CAST_HIP_FLAG_MFMA_F32;
     uint32_t code = fam;
     if (fam == CAST_HIP_VARIANT_TILE && c.second)
-      code = xcd ? (gauss ? kTileGaussXcd : kTileXcd)
+      code = xcd ? (swz ? (gauss ? kTileGaussXcdSwizzled : kTileXcdSwizzled)
+                        : (gauss ? kTileGaussXcd : kTileXcd))
                  : gauss ? (swz ? kTileGaussSwizzled : kTileGauss) : kTileSwizzled;
     if (fam == CAST_HIP_VARIANT_MFMA && f32_products)
       code = kMfmaF32;

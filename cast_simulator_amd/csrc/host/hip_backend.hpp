@@ -95,7 +95,7 @@ struct PlanInfo {
 };
 
 // Per-gate kernel autotuning: for a dense gate on a statevector of at least
-// `min_qubits` qubits, every applicable kernel (wave tile in three orders and
+// `min_qubits` qubits, every applicable kernel (wave tile in four orders and
 // two product forms, matrix core in three orders and, at f32 k >= 5, two
 // product forms, direct) is timed on the real gate, and the
 // fastest (by minimum) is used from then on.  Keyed by (kernel id, n).
@@ -121,6 +121,10 @@ constexpr uint32_t kMfmaF32 = 10;
 // (CAST_HIP_FLAG_TILE_SWIZZLE).
 constexpr uint32_t kMfmaXcd = 11;
 constexpr uint32_t kMfmaSwizzled = 12;
+// ... and both wave-tile forms in XCD-blocked order with the low tile bits
+// bit-reversed within each XCD's range (both flags).
+constexpr uint32_t kTileXcdSwizzled = 13;
+constexpr uint32_t kTileGaussXcdSwizzled = 14;
 
 struct TuneEntry {
   KernelId id = 0;
