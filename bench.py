@@ -36,17 +36,17 @@ KERNEL = {1: "k_dense", 2: "k_monomial", 3: "k_tile", 4: "k_mfma", 5: "k_tile(sw
           6: "k_tile(3 products)", 7: "k_tile(3 products, swizzled)", 8: "k_tile(xcd order)",
           9: "k_tile(3 products, xcd order)", 10: "k_mfma(f32 products)",
           11: "k_mfma(xcd order)", 12: "k_mfma(swizzled)", 13: "k_tile(xcd order, swizzled)",
-          14: "k_tile(3 products, xcd order, swizzled)"}
+          14: "k_tile(3 products, xcd order, swizzled)", 15: "k_mfma(xcd order, swizzled)"}
 
 
 def mfma_symbol(f: int, ctype: str, k: int) -> str:
     """Profiler symbol of a matrix-core or monomial launch: f32 k = 5, 6 run
     the bf16 split-product kernel unless the f32-product kernel (code 10) was
-    chosen; codes 11 and 12 are the default kernel in XCD-blocked and in
-    swizzled tile order (same symbol)."""
-    if f in (4, 11, 12) and ctype == "float" and k >= 5:
+    chosen; codes 11, 12 and 15 are the default kernel in XCD-blocked,
+    swizzled and both tile orders (same symbol)."""
+    if f in (4, 11, 12, 15) and ctype == "float" and k >= 5:
         return f"k_mfma_split<{k}>"
-    if f in (4, 10, 11, 12):
+    if f in (4, 10, 11, 12, 15):
         return f"k_mfma6<{ctype}>" if k == 6 else f"k_mfma<{ctype}, {k}>"
     return f"{KERNEL[f]}<{ctype}, {k}>"
 
@@ -305,7 +305,7 @@ def run_dense_large(cs, n64: int = 33, n32: int = 34, depth: int = 12):
             k = info[kid].k
             dense = info[kid].scalar_nnz == 2 * (1 << k) ** 2
             f = fam[kid]
-            if f in (4, 10, 11, 12):
+            if f in (4, 10, 11, 12, 15):
                 kern = mfma_symbol(f, ctype, k)
             elif f == 3 and (k == 4 or (k == 5 and prec == "f32")):
                 # automatic wave tiles take the three-product form here

@@ -174,6 +174,7 @@ HipKernelManager::TuneState &HipKernelManager::tune_state_locked(KernelId id, ui
       {CAST_HIP_VARIANT_MFMA, CAST_HIP_FLAG_MFMA_F32},
       {CAST_HIP_VARIANT_MFMA, CAST_HIP_FLAG_TILE_XCD},
       {CAST_HIP_VARIANT_MFMA, CAST_HIP_FLAG_TILE_SWIZZLE},
+      {CAST_HIP_VARIANT_MFMA, CAST_HIP_FLAG_TILE_XCD | CAST_HIP_FLAG_TILE_SWIZZLE},
       {CAST_HIP_VARIANT_DENSE, 0u}};
   for (const auto &c : cands) {
     cast_hip_kernel_gen_spec_t s{};
@@ -200,18 +201,16 @@ HipKernelManager::TuneState &HipKernelManager::tune_state_locked(KernelId id, ui
                  : gauss ? (swz ? kTileGaussSwizzled : kTileGauss) : kTileSwizzled;
     if (fam == CAST_HIP_VARIANT_MFMA && f32_products)
       code = kMfmaF32;
-    if (fam == CAST_HIP_VARIANT_MFMA && xcd)
-      code = kMfmaXcd;
-    if (fam == CAST_HIP_VARIANT_MFMA && swz)
-      code = kMfmaSwizzled;
+    if (fam == CAST_HIP_VARIANT_MFMA && (xcd || swz))
+      code = xcd ? (swz ? kMfmaXcdSwizzled : kMfmaXcd) : kMfmaSwizzled;
     const bool gauss_ok = g.n_qubits() == 4 || g.n_qubits() == 5;
     // f32 products differ from the default matrix-core kernel only at f32, k >= 5
     const bool f32_products_ok = spec_.precision == CAST_PRECISION_F32 && g.n_qubits() >= 5 &&
                                  fam == CAST_HIP_VARIANT_MFMA;
     const bool flags_ok = !c.second || (fam == CAST_HIP_VARIANT_TILE && !f32_products) ||
                           (f32_products && f32_products_ok) ||
-                          (fam == CAST_HIP_VARIANT_MFMA && (c.second == CAST_HIP_FLAG_TILE_XCD ||
-                                                            c.second == CAST_HIP_FLAG_TILE_SWIZZLE));
+                          (fam == CAST_HIP_VARIANT_MFMA &&
+                           (c.second & ~(CAST_HIP_FLAG_TILE_XCD | CAST_HIP_FLAG_TILE_SWIZZLE)) == 0);
     if (!ok || !flags_ok || (gauss && !gauss_ok) ||
         std::find(st.fams.begin(), st.fams.end(), code) != st.fams.end()) {
       cast_hip_gate_plan_destroy(plan); // inapplicable here, or same kernel as another

@@ -96,7 +96,7 @@ struct PlanInfo {
 
 // Per-gate kernel autotuning: for a dense gate on a statevector of at least
 // `min_qubits` qubits, every applicable kernel (wave tile in four orders and
-// two product forms, matrix core in three orders and, at f32 k >= 5, two
+// two product forms, matrix core in four orders and, at f32 k >= 5, two
 // product forms, direct) is timed on the real gate, and the
 // fastest (by minimum) is used from then on.  Keyed by (kernel id, n).
 // Successive halving: every candidate is timed once; candidates slower than
@@ -125,6 +125,8 @@ constexpr uint32_t kMfmaSwizzled = 12;
 // bit-reversed within each XCD's range (both flags).
 constexpr uint32_t kTileXcdSwizzled = 13;
 constexpr uint32_t kTileGaussXcdSwizzled = 14;
+// ... and the matrix-core kernel with both orders.
+constexpr uint32_t kMfmaXcdSwizzled = 15;
 
 struct TuneEntry {
   KernelId id = 0;

@@ -194,14 +194,15 @@ VARIANTS = [(abi.VARIANT_AUTO, 0), (abi.VARIANT_DENSE, 0), (abi.VARIANT_TILE, 0)
             (abi.VARIANT_TILE, abi.FLAG_TILE_GAUSS | abi.FLAG_TILE_XCD),
             (abi.VARIANT_MFMA, 0), (abi.VARIANT_MFMA, abi.FLAG_MFMA_F32),
             (abi.VARIANT_MFMA, abi.FLAG_TILE_XCD), (abi.VARIANT_MFMA, abi.FLAG_TILE_SWIZZLE),
-            (abi.VARIANT_TILE, abi.FLAG_TILE_XCD | abi.FLAG_TILE_SWIZZLE)]
+            (abi.VARIANT_TILE, abi.FLAG_TILE_XCD | abi.FLAG_TILE_SWIZZLE),
+            (abi.VARIANT_MFMA, abi.FLAG_TILE_XCD | abi.FLAG_TILE_SWIZZLE)]
 
 
 @pytest.mark.parametrize("variant,flags", VARIANTS,
                          ids=["auto", "direct", "tile", "tile-swizzled", "tile-gauss",
                               "tile-gauss-swizzled", "tile-xcd", "tile-gauss-xcd", "mfma",
                               "mfma-f32-products", "mfma-xcd", "mfma-swizzled",
-                              "tile-xcd-swizzled"])
+                              "tile-xcd-swizzled", "mfma-xcd-swizzled"])
 @pytest.mark.parametrize("f32", [False, True], ids=["f64", "f32"])
 def test_every_kernel_family_every_shape(f32, variant, flags):
     """Each dense kernel family, forced through the C ABI, on every target
@@ -450,8 +451,9 @@ def test_autotuned_kernels_match_oracle(f32):
     # three products (6), both (7), XCD order (8), three products in XCD order
     # (9), matrix core with f32 products at f32 k >= 5 (10), matrix core in XCD
     # order (11) and in swizzled order (12), wave tiles in XCD order with the
-    # low tile bits bit-reversed (13) and the same in three products (14)
-    assert len(report) == len(ids) and all(e["chosen"] in range(1, 15) and e["chosen"] != 2
+    # low tile bits bit-reversed (13) and the same in three products (14), matrix
+    # core in both orders (15)
+    assert len(report) == len(ids) and all(e["chosen"] in range(1, 16) and e["chosen"] != 2
                                            for e in report)
     for e in report:
         codes = [c for c, _ in e["best_s"]]
@@ -498,7 +500,7 @@ ROUNDTRIP_SHAPES = [[0, 1, 2, 3, 4], [5, 11, 17, 23, 29], [1, 9, 15, 26], [3, 6,
                          ids=["direct", "tile", "tile-swizzled", "tile-gauss",
                               "tile-gauss-swizzled", "tile-xcd", "tile-gauss-xcd", "mfma",
                               "mfma-f32-products", "mfma-xcd", "mfma-swizzled",
-                              "tile-xcd-swizzled"])
+                              "tile-xcd-swizzled", "mfma-xcd-swizzled"])
 @pytest.mark.parametrize("f32", [False, True], ids=["f64", "f32"])
 def test_roundtrip_at_30_qubits_every_family(f32, variant, flags):
     """Each kernel family forced through the C ABI at the headline size: the
