@@ -20,6 +20,15 @@
 //                identity row, so e.g. CZ / CP move 1/4 of the bytes.
 //   k_dense_batch  k_dense over many (matrix, statevector) items, item =
 //                blockIdx.z (cast_hip_gate_apply_batch).
+//   k_tile, k_tile3, k_mfma, k_mfma6, k_mfma_split: see their sections.
+//
+// Tile orders (wave tiles and matrix-core tiles): launch order, the low
+// tile-index bits bit-reversed, XCD-contiguous (workgroups are dispatched
+// to the 8 XCDs round-robin, so renumbering them gives each XCD one
+// contiguous eighth of the tiles), or both.  Plan flags select them; the
+// host autotuner tries each per gate; launches chosen automatically use the
+// XCD order for wave tiles, f64 matrix cores and f32 split products from 32
+// qubits on.
 
 #include "cast_hip.h"
 #include "common.hpp"
