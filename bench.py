@@ -89,9 +89,11 @@ def run_gpu(cs, n: int, precision: str, steps: int, warmup: int, max_size: int):
     sv = cs.HipStatevector(n, cs.Precision.F64 if precision == "f64" else cs.Precision.F32)
     sv.zero()
     # Setup (untimed, like the reference's compile phase): per-gate kernel
-    # autotuning on the resident statevector.
+    # autotuning on the resident statevector, three timings per surviving
+    # candidate (two gave 5721-5724 GB/s, three 5729-5730 on the same box).
     t_tune = time.perf_counter()
-    rounds = mgr.tune(ids, sv, 32)
+    mgr.set_autotune(True, 22, 3)
+    rounds = mgr.tune(ids, sv, 48)
     t_tune = time.perf_counter() - t_tune
     fam = {i: mgr.family(i, n) for i in set(ids)}
     log(f"  autotune: {rounds} passes, {t_tune:.1f} s")
