@@ -450,7 +450,13 @@ PYBIND11_MODULE(_host, m) {
                              py::arg("max_s") = t.max_s);
            })
       .def("plan_info", &HipKernelManager::plan_info)
-      .def("family", &HipKernelManager::family)
+      .def("family", &HipKernelManager::family, py::arg("id"), py::arg("n_qubits"),
+           "Kernel a launch of `id` on an n-qubit statevector runs: 1 direct, 2 monomial, "
+           "3 wave tile, 4 matrix core (the CAST_HIP_VARIANT_* codes), or the autotuner's "
+           "choice: 5 wave tile bit-reversed, 6 three-product wave tile, 7 both, 8 wave tile "
+           "in XCD order, 9 three-product in XCD order, 10 matrix core with f32 products, "
+           "11 matrix core in XCD order, 12 matrix core bit-reversed, 13 wave tile in XCD "
+           "order bit-reversed, 14 three-product likewise, 15 matrix core likewise.")
       .def("n_kernels", &HipKernelManager::n_kernels)
       .def("set_autotune", &HipKernelManager::set_autotune, py::arg("on"),
            py::arg("min_qubits") = 22, py::arg("samples") = 2)
